@@ -1,0 +1,281 @@
+/*
+ * clothcore.h -- C ABI of libclothcore, an MI355X (gfx950) cloth simulation
+ * core: CSR mass-spring Verlet step, Karras LBVH over a triangle body and
+ * point-vs-LBVH collision projection.
+ *
+ * The reference (sutongkui/simulator) has no library or FFI: its path sits
+ * behind the C++ classes Simulator (src/simulator.h:9-59), Springs
+ * (src/spring.h:14-36) and BVHAccel (src/bvh/bvh.h:20-101).  Each entry point
+ * below names the reference member it replaces.  Plain pointers and sizes
+ * only.  Every call returns CC_OK (0) or an error code; cc_last_error()
+ * returns a thread-local message (the reference prints and continues,
+ * src/common.h:4-10).
+ *
+ * Memory: host pointers unless a name says "_dev".  Arrays returned through
+ * cc_csr are allocated by the library and released with cc_csr_free().
+ * A cc_sim is single-threaded; separate handles may run on separate streams.
+ */
+#ifndef CLOTHCORE_H
+#define CLOTHCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC_OK 0
+#define CC_ERR_INVALID 1     /* bad argument or size                       */
+#define CC_ERR_HIP 2         /* HIP runtime error (no GPU, launch failure) */
+#define CC_ERR_ALLOC 3       /* host or device allocation failed           */
+#define CC_ERR_NONFINITE 4   /* NaN / Inf detected in the state            */
+#define CC_ERR_UNSUPPORTED 5 /* option not available in this build         */
+
+/* spring record, s_spring in src/spring.h:8-12 (8 bytes) */
+typedef struct cc_spring {
+  int32_t end;      /* other end of the spring (vertex index) */
+  float original;   /* rest length                            */
+} cc_spring;
+
+/* one CSR spring set: row[n+1] offsets into col[nnz]
+   (Springs::CSR_spring output, src/spring.cpp:374-391) */
+typedef struct cc_csr {
+  uint32_t* row;
+  cc_spring* col;
+  uint32_t n;
+  uint32_t nnz;
+} cc_csr;
+
+/* Physics and LBVH constants.  cc_params_default() gives the reference
+   values: verlet.cu:13-21 (dt=1/40, mass=0.3, g=(0,-0.00981,0), damp=-0.02,
+   k = {100 structure, 2 bend}), verlet.cu:62 (kd=-0.5), verlet.cu:73
+   (eps=FLT_EPSILON), primitive.cu:44 (box depth 0.01) and
+   simulation.cu:132 (body extend 0.003). */
+typedef struct cc_params {
+  float dt;
+  float mass;
+  float gravity[3];
+  float damp;         /* force += vel * damp                         */
+  float kd;           /* spring damping coefficient                  */
+  float k[3];         /* stiffness of spring type 0, 1, 2            */
+  float rest_scale;   /* rest length multiplier (1 in the reference) */
+  float eps;          /* bias added to every deltaP component        */
+  float box_depth;    /* expanded triangle box depth                 */
+  float body_extend;  /* Mesh::vertex_extend distance at body set-up */
+} cc_params;
+
+void cc_params_default(cc_params* p);
+
+/* Library version and last error (thread-local, never NULL). */
+int cc_version(void);
+const char* cc_last_error(void);
+/* Number of visible HIP devices (0 when no GPU); never initialises a context
+   beyond the runtime's own device enumeration. */
+int cc_device_count(int* count);
+
+/* ---------------------------------------------------------------- springs */
+
+/* Springs::Springs + CSR_structure_spring + CSR_bend_spring
+   (spring.cpp:82-91, 286-391): structure = unique 1-ring neighbours from
+   face edges, bend = vertices opposite each shared edge.  verts4 is
+   Mesh::vertices (vec4, w ignored), faces is Mesh::vertex_indices (3 per
+   face).  Rest length = distance at set-up. */
+int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                         cc_csr* out_structure, cc_csr* out_bend);
+
+/* Grid cloth springs: type 0 structural (4-neighbour), type 1 shear (both
+   diagonals), type 2 bend (skip-one).  Vertex (i, j) is index j*nx + i;
+   rows list their ends in ascending index order; rest = distance. */
+int cc_springs_grid(uint32_t nx, uint32_t nz, const float* pos3, cc_csr out[3]);
+
+void cc_csr_free(cc_csr* csr);
+
+/* --------------------------------------------------------- scene helpers */
+/* Synthetic inputs (the reference loads OBJ files; configs are synthetic).
+   All seeded with splitmix64 so host and GPU box produce identical data. */
+
+/* x-z grid at height y0: x = i*spacing, z = -0.5 + j*spacing, jitter each
+   coordinate by a seeded uniform in [-jitter, jitter].  pos3[3*nx*nz]. */
+int cc_scene_grid_cloth(uint32_t nx, uint32_t nz, double spacing, double y0, double jitter,
+                        uint64_t seed, float* pos3);
+
+/* Icosphere (outward CCW faces).  Sizes: V = 10*4^level + 2, F = 20*4^level.
+   normals3 (optional) receives unit vertex normals. */
+int cc_scene_icosphere(uint32_t level, double radius, const double center[3], float* verts3,
+                       uint32_t* tris, float* normals3);
+
+/* Capsule-union body description used by the per-frame deformation. */
+typedef struct cc_capsules {
+  uint32_t ncaps;
+  uint32_t* vstart;   /* [ncaps+1] vertex range of each capsule   */
+  float* p0;          /* [3*ncaps] axis start                      */
+  float* p1;          /* [3*ncaps] axis end                        */
+  float* radius;      /* [ncaps]                                   */
+  float* phase;       /* [ncaps] breathing phase (radians)         */
+} cc_capsules;
+
+/* Seeded union of ncaps capsules with radii in [rmin, rmax] inside the box
+   [lo, hi], tessellated with nseg segments around and nring rings per
+   hemisphere (+ ncyl cylinder rings).  Query sizes with verts3 == NULL:
+   *nv and *nf are always written. caps (optional) receives the layout;
+   arrays in caps must hold ncaps (+1) entries. normals3 optional. */
+int cc_scene_capsule_body(uint32_t ncaps, double rmin, double rmax, const double lo[3],
+                          const double hi[3], uint32_t nseg, uint32_t nring, uint32_t ncyl,
+                          uint64_t seed, float* verts3, uint32_t* tris, float* normals3,
+                          uint32_t* nv, uint32_t* nf, cc_capsules* caps);
+
+/* Mesh::vertex_extend (Mesh.cpp:101-117): move each vertex by dist along
+   the normal of the first face corner that visits it.  normal_index may be
+   NULL, meaning normal index == vertex index. stride = 3 or 4 floats. */
+int cc_mesh_vertex_extend(float* verts, uint32_t stride, uint32_t nv, const uint32_t* faces,
+                          uint32_t nf, const uint32_t* normal_index, const float* normals3,
+                          float dist);
+
+/* --------------------------------------------------------------- simulator */
+
+typedef struct cc_sim cc_sim;
+
+#define CC_BODY_NONE 0     /* no collider                                   */
+#define CC_BODY_STATIC 1   /* LBVH built once (the reference's behaviour)   */
+#define CC_BODY_REFIT 2    /* body deformed per frame, boxes refitted       */
+#define CC_BODY_REBUILD 3  /* body deformed per frame, LBVH rebuilt         */
+
+/* Cloth: Mesh::vertices (vec4) plus 1..3 typed CSR spring sets in the
+   order their forces are summed (verlet.cu:148-149). */
+typedef struct cc_cloth_desc {
+  uint32_t n;
+  const float* verts4;
+  uint32_t ntypes;
+  const uint32_t* row[3];
+  const cc_spring* col[3];
+} cc_cloth_desc;
+
+/* Body: vertices (stride 3 or 4 floats) and triangles.  For REFIT and
+   REBUILD, caps describes the capsule decomposition that the deformation
+   breathes; motion: translation amp*sin(2 pi f t)*dir, rotation about the
+   y axis through pivot by rot_amp*sin(2 pi f t), radial breathing
+   1 + breath_amp*sin(2 pi breath_freq t + phase_k), t = frame*dt. */
+typedef struct cc_body_desc {
+  uint32_t nv;
+  const float* verts;
+  uint32_t stride;
+  uint32_t nf;
+  const uint32_t* tris;
+  uint32_t mode;
+  uint32_t extend;         /* 1: apply vertex_extend(body_extend) first */
+  const float* normals3;   /* per-vertex normals for extend (or NULL)   */
+  const cc_capsules* caps; /* required for REFIT / REBUILD              */
+  double amp, freq, dir[3], rot_amp, pivot[3], breath_amp, breath_freq;
+} cc_body_desc;
+
+/* Simulator::Simulator(Mesh& cloth, Mesh& body) (simulation.cu:47-52):
+   uploads the cloth (x_last = x_cur, contact = 0, simulation.cu:54-100),
+   the springs (init_spring, :102-125) and builds the body LBVH
+   (build_bvh, :127-138).  body may be NULL (no collider). */
+int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* body, const cc_params* params,
+                  cc_sim** out);
+int cc_sim_destroy(cc_sim* sim);
+
+/* Simulator::simulate (simulation.cu:141-150) for nsteps frames: per frame
+   the body update (REFIT / REBUILD), then the fused Verlet + spring +
+   collision kernel (verlet.cu:129-161), then the ping-pong swap
+   (:243-251).  Asynchronous on stream (NULL: the simulator's own stream);
+   steps are replayed from a captured hipGraph. */
+int cc_sim_step(cc_sim* sim, int nsteps, void* stream);
+int cc_sim_synchronize(cc_sim* sim);
+
+/* Simulator::update_vertex (simulation.cu:253-257): x_cur = x_last = xyz. */
+int cc_sim_update_vertex(cc_sim* sim, uint32_t idx, const float xyz[3]);
+
+/* Pin vertices at their current position (x_last = x_cur, zero velocity);
+   pinned vertices keep that position, store a zero contact normal and skip
+   collision.  Not in the reference; required by the configurations. */
+int cc_sim_set_pins(cc_sim* sim, const uint32_t* idx, uint32_t k);
+
+/* Replace physics parameters (takes effect at the next step). */
+int cc_sim_set_params(cc_sim* sim, const cc_params* params);
+
+/* Host copies of x_cur / x_last / contact normal, AoS vec3 like the
+   reference buffers (simulator.h:41-50).  Any pointer may be NULL. */
+int cc_sim_get_state(cc_sim* sim, float* x_cur, float* x_prev, float* contact_n);
+int cc_sim_set_state(cc_sim* sim, const float* x_cur, const float* x_prev, const float* contact_n);
+int cc_sim_frame(cc_sim* sim, uint64_t* frame);
+int cc_sim_set_frame(cc_sim* sim, uint64_t frame);
+
+/* Debug outputs of the next steps: pre-projection positions (vec3), the
+   sorted leaf tested (-1 none) and the inside flag of the last step.  Off by
+   default; enabling costs extra bytes per step. */
+int cc_sim_set_debug(cc_sim* sim, int enable);
+int cc_sim_get_debug(cc_sim* sim, float* pre3, int32_t* hit, uint8_t* inside);
+
+/* Nodes-visited counter of the collision traversal (summed since reset). */
+int cc_sim_set_counting(cc_sim* sim, int enable);
+int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries);
+
+/* Finite check of x_cur: CC_ERR_NONFINITE if any NaN / Inf. */
+int cc_sim_check_finite(cc_sim* sim);
+
+/* ------------------------------------------------------------------ body */
+
+/* Replace the body vertices (stride 3 floats, rest pose for the deformation)
+   and rebuild the LBVH from them. */
+int cc_body_set_vertices(cc_sim* sim, const float* verts3);
+/* Current (deformed) body vertices, vec3. */
+int cc_body_get_vertices(cc_sim* sim, float* verts3);
+/* The float parameters the deformation used for the last body update:
+   R[9] row-major, T[3] (pivot folded in), breath[ncaps]. */
+int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath);
+/* BVHAccel construction (bvh.cu:264-297) / refit (calculateBoudingBox,
+   cudaBRT.cu:166-206) over the current body vertices. */
+int cc_bvh_rebuild(cc_sim* sim);
+int cc_bvh_refit(cc_sim* sim);
+int cc_bvh_size(cc_sim* sim, uint32_t* leaves);
+
+/* Export the LBVH in the reference encoding (cudaBRT.h:10-33): childA/B
+   > 0 internal v-1, < 0 leaf -(v+1); parent v+1, 0 = root.  keys are the
+   sorted (morton << 32 | primitive) keys, order the primitive of each leaf,
+   boxes are [min xyz, max xyz]; root box of the whole tree in root6.
+   Sizes: leaves m, internal m-1.  Any pointer may be NULL. */
+int cc_bvh_export(cc_sim* sim, uint64_t* keys, uint32_t* order, int32_t* childA, int32_t* childB,
+                  int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf,
+                  float* root6);
+
+/* Collision query + projection for n points (collision_response_projection,
+   verlet.cu:24-50, without the Verlet part): hit = sorted leaf tested
+   (-1 none), inside = projection applied, out_pos / out_n projected point
+   and normal (zero when not inside). */
+int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, uint8_t* inside,
+                      float* out_pos, float* out_n);
+
+/* ------------------------------------------------------------------- vbo */
+
+/* cuda_update_vbo (simulation.cu:196-231) into a device buffer with the
+   reference VBO layout [vec4 pos (w=1) x n][vec2 uv x n_tex][vec3 normal x n]
+   (scene.cpp:115-123).  Vertex normal = normalised sum of adjacent face
+   normals (verlet.cu:91-116, 163-181); faces[3*nf] are the cloth indices.
+   lag = 1 reproduces the reference's use of the previous positions for face
+   normals (simulation.cu:221).  uv bytes are left untouched. */
+int cc_sim_set_faces(cc_sim* sim, const uint32_t* faces, uint32_t nf);
+int cc_sim_write_vbo(cc_sim* sim, void* dst_dev, size_t bytes, uint32_t n_tex, int lag);
+/* Host copy of the vertex normals of the last cc_sim_write_vbo. */
+int cc_sim_get_normals(cc_sim* sim, float* normals3);
+
+/* ----------------------------------------------------------------- timing */
+
+/* Per-kernel timing of nsteps steps with HIP events on the simulator
+   stream (each kernel bracketed), averaged per launch in microseconds. */
+typedef struct cc_timing {
+  double step_us;      /* whole step (graph replay), per step           */
+  double verlet_us;    /* fused Verlet + spring + collision kernel      */
+  double body_us;      /* deformation kernel                            */
+  double refit_us;     /* refit kernel                                  */
+  double build_us;     /* rebuild kernels (boxes, sort, Karras) summed  */
+  uint32_t steps;
+} cc_timing;
+int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+// cc_device.h -- device routines shared by the step, query and batched
+// kernels: first-hit LBVH traversal and the triangle plane test with
+// projection.  All comparisons and roundings follow the reference order so
+// the colliding-vertex set is bit-exact given the same query points.
+#pragma once
+
+#include "cc_internal.h"
+
+namespace cc {
+
+// D_BVH::intersect (cudaBRT.cu:239-285).  At each internal node both child
+// boxes are tested; an overlapped LEAF child is returned at once (A before
+// B); otherwise overlapped internal children are descended left first with
+// the right one deferred on a per-lane stack.  The stack never holds more
+// entries than the node depth, and the depth of a Karras tree over unique
+// 64-bit keys is below 64, so 64 entries cannot overflow.
+// Returns the sorted leaf index, or -1.  *visits counts internal nodes.
+__device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits) {
+  if (b.m == 0) return -1;
+  if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
+  int32_t stack[64];
+  int sp = 0;
+  int32_t node = 0;
+  // A well-formed tree visits each internal node at most once; the bound
+  // only guards against a corrupted tree (the loop always terminates).
+  for (uint32_t guard = 0; guard < b.m; ++guard) {
+    ++visits;
+    const float4* nd = reinterpret_cast<const float4*>(b.nodes + node);
+    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
+    const int4 L = reinterpret_cast<const int4*>(nd)[3];
+    // A = [q0.xyz, (q0.w, q1.x, q1.y)], B = [(q1.z, q1.w, q2.x), q2.yzw]
+    const bool oL = (p.x >= q0.x && p.x <= q0.w) && (p.y >= q0.y && p.y <= q1.x) &&
+                    (p.z >= q0.z && p.z <= q1.y);
+    const bool oR = (p.x >= q1.z && p.x <= q2.y) && (p.y >= q1.w && p.y <= q2.z) &&
+                    (p.z >= q2.x && p.z <= q2.w);
+    const int32_t ca = L.x, cb = L.y;
+    if (oL && ca < 0) return ~ca;
+    if (oR && cb < 0) return ~cb;
+    const bool tL = oL && ca >= 0, tR = oR && cb >= 0;
+    if (tL) {
+      if (tR) {
+        if (sp == 64) return -1;   // unreachable for depth < 64
+        stack[sp++] = cb;
+      }
+      node = ca;
+    } else if (tR) {
+      node = cb;
+    } else {
+      if (sp == 0) return -1;
+      node = stack[--sp];
+    }
+  }
+  return -1;
+}
+
+// Primitive::d_intersect (primitive.cu:70-85): n = normalize(cross(v1-v0,
+// v2-v0)), d = dot(p - v0, n); inside iff !(d > 0) (a NaN d is inside).
+__device__ inline bool plane_test(const float* verts, uint4 t, f3 p, float& d, f3& n) {
+  const f3 a = ld3(verts + 3 * (size_t)t.x), b = ld3(verts + 3 * (size_t)t.y),
+           c = ld3(verts + 3 * (size_t)t.z);
+  n = normalize(cross(b - a, c - a));
+  d = dot(p - a, n);
+  return !(d > 0.0f);
+}
+
+// collision_response_projection (verlet.cu:24-50) on a position p: when the
+// first-hit leaf's plane test says inside, p += |d| n and the contact normal
+// is n, else the contact normal is 0.  Returns true when projected.
+__device__ inline bool collide_project(const BvhView& b, f3& p, f3& n_out, int32_t& hit,
+                                       uint32_t& visits) {
+  hit = bvh_first_hit(b, p, visits);
+  n_out = mk(0.0f, 0.0f, 0.0f);
+  if (hit < 0) return false;
+  float d;
+  f3 n;
+  if (!plane_test(b.verts, b.leaf_tri[hit], p, d, n)) return false;
+  const float dd = 1.0f * fabsf(d);
+  p = p + n * dd;
+  n_out = n;
+  return true;
+}
+
+}  // namespace cc

@@ -1,0 +1,518 @@
+// lbvh.hip -- body deformation and the Karras LBVH on gfx950: expanded
+// triangle boxes, 30-bit Morton codes, LSD radix sort (LDS histogram +
+// wavefront-ballot ranking), clz64 radix-tree build and atomic refit.
+//
+// Reference path: BVHAccel (bvh.cu:154-297) computes boxes and codes on the
+// GPU, sorts on the host (Utilities.h:22-67) and builds the tree with
+// processInternalNode / calculateBoudingBox (cudaBRT.cu:120-206).  Here every
+// stage stays on the device; keys are (morton << 32 | primitive) so all
+// primitives are kept and the order is total.
+#include <algorithm>
+#include <cstring>
+
+#include "cc_body.h"
+
+namespace cc {
+
+// ------------------------------------------------------------ deformation
+// One block computes the frame's float parameters in LDS (double math, one
+// rounding to float), then every thread moves one vertex: radial breathing
+// about the closest point of its capsule axis, then R * Y + T.
+struct MotionArgs {
+  double amp, freq, dx, dy, dz, rot_amp, px, py, pz, breath_amp, breath_freq, dt;
+};
+
+constexpr int MAX_CAPS = 64;
+
+__global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, float* __restrict__ out,
+                                                uint32_t nv, uint32_t ncaps,
+                                                const uint32_t* __restrict__ vstart,
+                                                const float* __restrict__ p0, const float* __restrict__ p1,
+                                                const float* __restrict__ phase, MotionArgs a,
+                                                const uint64_t* __restrict__ frame_dev,
+                                                float* __restrict__ motion_out) {
+  __shared__ float sR[9], sT[3], sB[MAX_CAPS], sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
+  __shared__ uint32_t sV[MAX_CAPS + 1];
+  const uint32_t tid = threadIdx.x;
+  const double PI2 = 6.283185307179586476925;
+  const double t = (double)(*frame_dev) * a.dt;
+  if (tid == 0) {
+    const double w = sin(PI2 * a.freq * t);
+    const double th = a.rot_amp * w, c = cos(th), s = sin(th);
+    // rotation about +y: [[c,0,s],[0,1,0],[-s,0,c]]
+    const double R[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
+    const double P[3] = {a.px, a.py, a.pz};
+    const double D[3] = {a.dx, a.dy, a.dz};
+    for (int r = 0; r < 3; ++r) {
+      double rp = R[3 * r] * P[0] + R[3 * r + 1] * P[1] + R[3 * r + 2] * P[2];
+      sT[r] = (float)(P[r] - rp + a.amp * w * D[r]);
+    }
+    for (int q = 0; q < 9; ++q) sR[q] = (float)R[q];
+  }
+  for (uint32_t k = tid; k < ncaps; k += blockDim.x) {
+    sB[k] = (float)(1.0 + a.breath_amp * sin(PI2 * a.breath_freq * t + (double)phase[k]));
+    for (int q = 0; q < 3; ++q) {
+      sP0[3 * k + q] = p0[3 * k + q];
+      sP1[3 * k + q] = p1[3 * k + q];
+    }
+  }
+  for (uint32_t k = tid; k <= ncaps; k += blockDim.x) sV[k] = vstart[k];
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    if (tid < 9) motion_out[tid] = sR[tid];
+    if (tid < 3) motion_out[9 + tid] = sT[tid];
+    for (uint32_t k = tid; k < ncaps; k += blockDim.x) motion_out[12 + k] = sB[k];
+  }
+  const uint32_t v = blockIdx.x * blockDim.x + tid;
+  if (v >= nv) return;
+  // capsule of v: last k with vstart[k] <= v
+  uint32_t lo = 0, hi = ncaps - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (sV[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  const uint32_t k = lo;
+  const f3 X = ld3(rest + 3 * (size_t)v);
+  const f3 P0 = ld3(sP0 + 3 * k), P1 = ld3(sP1 + 3 * k);
+  const f3 d = P1 - P0;
+  const float dd = dot(d, d);
+  float tt = 0.0f;
+  if (dd > 0.0f) tt = dot(X - P0, d) / dd;
+  tt = fminf(fmaxf(tt, 0.0f), 1.0f);
+  const f3 A = P0 + d * tt;
+  const f3 Y = A + (X - A) * sB[k];
+  float* o = out + 3 * (size_t)v;
+  o[0] = ((sR[0] * Y.x + sR[1] * Y.y) + sR[2] * Y.z) + sT[0];
+  o[1] = ((sR[3] * Y.x + sR[4] * Y.y) + sR[5] * Y.z) + sT[1];
+  o[2] = ((sR[6] * Y.x + sR[7] * Y.y) + sR[8] * Y.z) + sT[2];
+}
+
+// ------------------------------------------------------------ boxes + codes
+constexpr int BOX_THREADS = 256;
+
+__device__ inline void box_sentinel(float* mn, float* mx) {   // bbox.h:17-22
+  mn[0] = mn[1] = mn[2] = 100.0f;
+  mx[0] = mx[1] = mx[2] = -100.0f;
+}
+
+// expanded box per triangle + one partial root box per block
+__global__ __launch_bounds__(BOX_THREADS) void k_prim_boxes(const float* __restrict__ verts,
+                                                            const uint32_t* __restrict__ tris, uint32_t m,
+                                                            float depth, float* __restrict__ boxes,
+                                                            float* __restrict__ partial) {
+  __shared__ float red[6][BOX_THREADS];
+  const uint32_t i = blockIdx.x * BOX_THREADS + threadIdx.x;
+  float mn[3], mx[3];
+  box_sentinel(mn, mx);
+  if (i < m) {
+    const uint32_t* t = tris + 3 * (size_t)i;
+    prim_box(ld3(verts + 3 * (size_t)t[0]), ld3(verts + 3 * (size_t)t[1]), ld3(verts + 3 * (size_t)t[2]),
+             depth, mn, mx);
+    float* b = boxes + 6 * (size_t)i;
+    b[0] = mn[0]; b[1] = mn[1]; b[2] = mn[2];
+    b[3] = mx[0]; b[4] = mx[1]; b[5] = mx[2];
+  }
+  for (int c = 0; c < 3; ++c) {
+    red[c][threadIdx.x] = mn[c];
+    red[3 + c][threadIdx.x] = mx[c];
+  }
+  __syncthreads();
+  for (int s = BOX_THREADS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < (uint32_t)s)
+      for (int c = 0; c < 3; ++c) {
+        red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+        red[3 + c][threadIdx.x] = fmaxf(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[6 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// root box (sentinel U all partials, recomputed per block) then the code of
+// each box centroid (compute_morton_bbox, bvh.cu:71-79)
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ boxes, uint32_t m,
+                                                const float* __restrict__ partial, uint32_t npart,
+                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
+                                                float* __restrict__ root_out) {
+  __shared__ float red[6][256];
+  float mn[3], mx[3];
+  box_sentinel(mn, mx);
+  for (uint32_t q = threadIdx.x; q < npart; q += 256)
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fminf(mn[c], partial[6 * (size_t)q + c]);
+      mx[c] = fmaxf(mx[c], partial[6 * (size_t)q + 3 + c]);
+    }
+  for (int c = 0; c < 3; ++c) {
+    red[c][threadIdx.x] = mn[c];
+    red[3 + c][threadIdx.x] = mx[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < (uint32_t)s)
+      for (int c = 0; c < 3; ++c) {
+        red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+        red[3 + c][threadIdx.x] = fmaxf(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  const float rmin[3] = {red[0][0], red[1][0], red[2][0]};
+  const float rext[3] = {red[3][0] - red[0][0], red[4][0] - red[1][0], red[5][0] - red[2][0]};
+  if (blockIdx.x == 0 && threadIdx.x < 6) root_out[threadIdx.x] = red[threadIdx.x][0];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const float* b = boxes + 6 * (size_t)i;
+  codes[i] = box_morton(b, b + 3, rmin, rext);
+  idx[i] = i;
+}
+
+// ------------------------------------------------------------ radix sort
+// Stable LSD sort of (code, index) pairs by code, 8 bits per pass.  Because
+// the indices enter in ascending order and every pass is stable, the result
+// is ordered by the full 64-bit key (code << 32 | index).
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 4;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             int shift, uint32_t* __restrict__ hist,
+                                                             uint32_t nblk) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * SORT_TILE;
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const uint32_t e = base + r * SORT_THREADS + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of len counters by one 1024-thread block
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint32_t len) {
+  __shared__ uint32_t sums[1024];
+  const uint32_t per = (len + 1023) / 1024;
+  const uint32_t b = threadIdx.x * per, e = min(len, b + per);
+  uint32_t s = 0;
+  for (uint32_t i = b; i < e; ++i) s += data[i];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+// Scatter with stable ranks: within a wave, lanes holding the same digit are
+// matched with 8 ballots; waves are ordered through per-digit LDS counts.
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin, uint32_t n,
+                                                                int shift, const uint32_t* __restrict__ hist,
+                                                                uint32_t nblk, uint32_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout) {
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[SORT_THREADS / 64][256];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  goff[tid] = hist[(size_t)tid * nblk + blockIdx.x];
+  run[tid] = 0;
+  const uint32_t base = blockIdx.x * SORT_TILE;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    for (int q = 0; q < SORT_THREADS / 64; ++q) wcnt[q][tid] = 0;
+    __syncthreads();
+    const uint32_t e = base + r * SORT_THREADS + tid;
+    const bool valid = e < n;
+    const uint32_t key = valid ? kin[e] : 0u, val = valid ? vin[e] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t msk = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      msk &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = __popcll(msk & lt);
+    if (valid && rank == 0) wcnt[w][d] = __popcll(msk);
+    __syncthreads();
+    {
+      uint32_t s = run[tid];
+      for (int q = 0; q < SORT_THREADS / 64; ++q) {
+        const uint32_t c = wcnt[q][tid];
+        wcnt[q][tid] = s;
+        s += c;
+      }
+      run[tid] = s;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = goff[d] + wcnt[w][d] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict__ codes,
+                                                     const uint32_t* __restrict__ idx,
+                                                     const uint32_t* __restrict__ tris, uint32_t m,
+                                                     uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t p = idx[i];
+  keys[i] = ((uint64_t)codes[i] << 32) | p;
+  const uint32_t* t = tris + 3 * (size_t)p;
+  leaf_tri[i] = make_uint4(t[0], t[1], t[2], p);
+}
+
+// ------------------------------------------------------------ Karras build
+// processInternalNode / determineRange / findSplit (cudaBRT.cu:9-161) with
+// delta = clz64 of 64-bit keys (-1 outside [0, m)).
+__global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ keys, int32_t m,
+                                                Node* __restrict__ nodes, int32_t* __restrict__ leaf_parent) {
+  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m - 1) return;
+  const uint64_t ki = keys[i];
+  auto delta = [&](int64_t j) -> int {
+    if (j < 0 || j >= m) return -1;
+    return __clzll(ki ^ keys[j]);
+  };
+  const int d = (delta(i + 1) - delta(i - 1)) > 0 ? 1 : -1;
+  const int dmin = delta(i - d);
+  int64_t lmax = 2;
+  while (delta(i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (delta(i + (l + t) * d) > dmin) l += t;
+  const int32_t j = (int32_t)(i + l * d);
+  const int32_t first = min(i, j), last = max(i, j);
+  const uint64_t kf = keys[first];
+  const int common = __clzll(kf ^ keys[last]);
+  int32_t split = first, step = last - first;
+  do {
+    step = (step + 1) >> 1;
+    const int32_t ns = split + step;
+    if (ns < last && (kf >> (63 - common)) == (keys[ns] >> (63 - common))) split = ns;
+  } while (step > 1);
+  const int32_t ca = (split == first) ? ~split : split;
+  const int32_t cb = (split + 1 == last) ? ~(split + 1) : split + 1;
+  nodes[i].childA = ca;
+  nodes[i].childB = cb;
+  nodes[i].other = j;
+  if (ca < 0) leaf_parent[split] = i; else nodes[split].parent = i;
+  if (cb < 0) leaf_parent[split + 1] = i; else nodes[split + 1].parent = i;
+  if (i == 0) nodes[0].parent = -1;
+}
+
+// ------------------------------------------------------------ refit
+// calculateBoudingBox (cudaBRT.cu:166-206): one thread per leaf computes the
+// leaf box, stores it in its parent's child slot and arrives at the parent;
+// the second arrival unites both child slots (sentinel U A U B) and climbs.
+// The arrival is an agent-scope acq_rel atomic, so the sibling's slot store
+// is visible (the reference has no fence).  The second arrival resets the
+// counter for the next refit.
+__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ verts,
+                                               const uint4* __restrict__ leaf_tri,
+                                               const int32_t* __restrict__ leaf_parent, Node* nodes,
+                                               uint32_t* counters, uint32_t m, float depth,
+                                               float* __restrict__ root_out, float* __restrict__ leaf0_box) {
+  const uint32_t l = blockIdx.x * 256 + threadIdx.x;
+  if (l >= m) return;
+  const uint4 t = leaf_tri[l];
+  float mn[3], mx[3];
+  prim_box(ld3(verts + 3 * (size_t)t.x), ld3(verts + 3 * (size_t)t.y), ld3(verts + 3 * (size_t)t.z), depth,
+           mn, mx);
+  if (m == 1) {
+    float smn[3], smx[3];
+    box_sentinel(smn, smx);
+    for (int c = 0; c < 3; ++c) {
+      leaf0_box[c] = mn[c];
+      leaf0_box[3 + c] = mx[c];
+      root_out[c] = fminf(smn[c], mn[c]);
+      root_out[3 + c] = fmaxf(smx[c], mx[c]);
+    }
+    return;
+  }
+  int32_t child = ~(int32_t)l;
+  int32_t x = leaf_parent[l];
+  while (true) {
+    Node* nd = nodes + x;
+    float* slot = (nd->childA == child) ? nd->a_min : nd->b_min;   // min[3] then max[3]
+    slot[0] = mn[0]; slot[1] = mn[1]; slot[2] = mn[2];
+    slot[3] = mx[0]; slot[4] = mx[1]; slot[5] = mx[2];
+    const uint32_t old = __hip_atomic_fetch_add(counters + x, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    counters[x] = 0;
+    box_sentinel(mn, mx);
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fminf(fminf(mn[c], nd->a_min[c]), nd->b_min[c]);
+      mx[c] = fmaxf(fmaxf(mx[c], nd->a_max[c]), nd->b_max[c]);
+    }
+    child = x;
+    x = nd->parent;
+    if (x < 0) {
+      for (int c = 0; c < 3; ++c) {
+        root_out[c] = mn[c];
+        root_out[3 + c] = mx[c];
+      }
+      return;
+    }
+  }
+}
+
+// ------------------------------------------------------------ host side
+void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, uint32_t mode_, float depth_,
+                const cc_capsules* caps, const MotionDesc& md, hipStream_t s) {
+  CC_REQUIRE(mode_ <= CC_BODY_REBUILD, "body: unknown mode");
+  mode = mode_;
+  nv = nv_;
+  m = nf;
+  depth = depth_;
+  motion = md;
+  if (mode == CC_BODY_NONE) return;
+  CC_REQUIRE(v3 && t3 && nv > 0, "body: vertices and triangles required");
+  CC_REQUIRE(m < (1u << 30), "body: too many triangles");
+  for (size_t i = 0; i < 3 * (size_t)m; ++i) CC_REQUIRE(t3[i] < nv, "body: triangle index out of range");
+  rest.upload(v3, 3 * (size_t)nv, s);
+  verts.upload(v3, 3 * (size_t)nv, s);
+  tris.upload(t3, 3 * (size_t)m, s);
+  if (mode == CC_BODY_REFIT || mode == CC_BODY_REBUILD) {
+    CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
+    CC_REQUIRE(caps->vstart[0] == 0 && caps->vstart[caps->ncaps] == nv, "body: capsule ranges must cover vertices");
+    for (uint32_t k = 0; k < caps->ncaps; ++k)
+      CC_REQUIRE(caps->vstart[k] <= caps->vstart[k + 1], "body: capsule ranges must be ascending");
+    ncaps = caps->ncaps;
+    cap_vstart.upload(caps->vstart, ncaps + 1, s);
+    cap_p0.upload(caps->p0, 3 * (size_t)ncaps, s);
+    cap_p1.upload(caps->p1, 3 * (size_t)ncaps, s);
+    cap_phase.upload(caps->phase, ncaps, s);
+    motion_out.alloc(12 + ncaps);
+    motion_out.zero(s);
+  }
+  nblk_box = grid_for(m, BOX_THREADS);
+  boxes.alloc(6 * (size_t)m);
+  partial.alloc(6 * (size_t)nblk_box);
+  for (int b = 0; b < 2; ++b) {
+    code[b].alloc(m);
+    idx[b].alloc(m);
+  }
+  hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
+  keys.alloc(m);
+  leaf_tri.alloc(m);
+  leaf_parent.alloc(m);
+  nodes.alloc(std::max<uint32_t>(m, 2) - 1);
+  counters.alloc(std::max<uint32_t>(m, 2) - 1);
+  counters.zero(s);
+  nodes.zero(s);
+  root.alloc(6);
+  leaf0_box.alloc(6);
+  leaf0_box.zero(s);
+}
+
+void Body::deform(const uint64_t* frame_dev, double dt, hipStream_t s) {
+  if (nv == 0 || ncaps == 0) return;
+  MotionArgs a{motion.amp,      motion.freq,      motion.dir[0],    motion.dir[1],
+               motion.dir[2],   motion.rot_amp,   motion.pivot[0],  motion.pivot[1],
+               motion.pivot[2], motion.breath_amp, motion.breath_freq, dt};
+  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, verts.p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
+                                               cap_phase.p, a, frame_dev, motion_out.p);
+  CC_HIP(hipGetLastError());
+}
+
+void Body::build(hipStream_t s) {
+  if (mode == CC_BODY_NONE || m == 0) return;
+  k_prim_boxes<<<nblk_box, BOX_THREADS, 0, s>>>(verts.p, tris.p, m, depth, boxes.p, partial.p);
+  CC_HIP(hipGetLastError());
+  k_morton<<<grid_for(m, 256), 256, 0, s>>>(boxes.p, m, partial.p, nblk_box, code[0].p, idx[0].p, root.p);
+  CC_HIP(hipGetLastError());
+  const uint32_t nblk = grid_for(m, SORT_TILE);
+  int src = 0;
+  for (int shift = 0; shift < 32; shift += 8) {
+    k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(code[src].p, m, shift, hist.p, nblk);
+    k_scan<<<1, 1024, 0, s>>>(hist.p, 256 * nblk);
+    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(code[src].p, idx[src].p, m, shift, hist.p, nblk,
+                                                  code[src ^ 1].p, idx[src ^ 1].p);
+    CC_HIP(hipGetLastError());
+    src ^= 1;
+  }
+  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(code[src].p, idx[src].p, tris.p, m, keys.p, leaf_tri.p);
+  CC_HIP(hipGetLastError());
+  if (m >= 2) {
+    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(keys.p, (int32_t)m, nodes.p, leaf_parent.p);
+    CC_HIP(hipGetLastError());
+  }
+  refit(s);
+}
+
+void Body::refit(hipStream_t s) {
+  if (mode == CC_BODY_NONE || m == 0) return;
+  k_refit<<<grid_for(m, 256), 256, 0, s>>>(verts.p, leaf_tri.p, leaf_parent.p, nodes.p, counters.p, m, depth,
+                                             root.p, leaf0_box.p);
+  CC_HIP(hipGetLastError());
+}
+
+void Body::frame_update(const uint64_t* frame_dev, double dt, hipStream_t s) {
+  if (mode == CC_BODY_REFIT) {
+    deform(frame_dev, dt, s);
+    refit(s);
+  } else if (mode == CC_BODY_REBUILD) {
+    deform(frame_dev, dt, s);
+    build(s);
+  }
+}
+
+void Body::export_tree(uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB, int32_t* parent_int,
+                       int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6, hipStream_t s) {
+  CC_REQUIRE(mode != CC_BODY_NONE, "bvh export: no body");
+  std::vector<uint64_t> k(m);
+  std::vector<uint4> lt(m);
+  std::vector<int32_t> lp(m);
+  std::vector<Node> nd(m >= 2 ? m - 1 : 0);
+  float r[6], l0[6];
+  CC_HIP(hipMemcpyAsync(k.data(), keys.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri.p, m * sizeof(uint4), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lp.data(), leaf_parent.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (!nd.empty()) CC_HIP(hipMemcpyAsync(nd.data(), nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(r, root.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(l0, leaf0_box.p, sizeof(l0), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < m; ++i) {
+    if (okeys) okeys[i] = k[i];
+    if (order) order[i] = lt[i].w;
+  }
+  if (root6) std::memcpy(root6, r, sizeof(r));
+  if (m == 1) {
+    if (box_leaf) std::memcpy(box_leaf, l0, sizeof(l0));
+    if (parent_leaf) parent_leaf[0] = 0;
+    return;
+  }
+  auto slot_of = [&](int32_t par, int32_t child, float* dst) {
+    const Node& p = nd[par];
+    const float* src = (p.childA == child) ? p.a_min : p.b_min;
+    std::memcpy(dst, src, 6 * sizeof(float));
+  };
+  for (uint32_t x = 0; x + 1 < m; ++x) {
+    const Node& n = nd[x];
+    if (childA) childA[x] = n.childA >= 0 ? n.childA + 1 : n.childA;   // ~l == -(l+1)
+    if (childB) childB[x] = n.childB >= 0 ? n.childB + 1 : n.childB;
+    if (parent_int) parent_int[x] = n.parent + 1;
+    if (box_int) {
+      if (n.parent < 0) std::memcpy(box_int + 6 * (size_t)x, r, sizeof(r));
+      else slot_of(n.parent, (int32_t)x, box_int + 6 * (size_t)x);
+    }
+  }
+  for (uint32_t l = 0; l < m; ++l) {
+    if (parent_leaf) parent_leaf[l] = lp[l] + 1;
+    if (box_leaf) slot_of(lp[l], ~(int32_t)l, box_leaf + 6 * (size_t)l);
+  }
+}
+
+}  // namespace cc

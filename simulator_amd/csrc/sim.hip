@@ -1,0 +1,770 @@
+// sim.hip -- the cloth simulator: fused Verlet + CSR spring + collision step
+// (verlet.cu:52-161), graph-captured per-frame sequence, state I/O, VBO
+// writer and the C ABI of the simulator.
+//
+// Device layout (per vertex):
+//   state[2]  24 B {x_cur.xyz, x_prev.xyz} ping-pong; a neighbour gather
+//             reads one 24-byte record instead of two 12-byte vectors
+//   contact   12 B contact normal of the last step (verlet.cu:151)
+//   row       4 B merged CSR row start of all spring types; bit 31 = pinned
+//   rec       8 B per directed spring {end | type << 30, rest length}
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "cc_body.h"
+#include "cc_device.h"
+
+namespace cc {
+
+constexpr uint32_t PIN_BIT = 0x80000000u;
+constexpr uint32_t ROW_MASK = 0x7fffffffu;
+constexpr uint32_t END_MASK = 0x3fffffffu;
+
+struct StepArgs {
+  float inv_dt, dt2, inv_mass;
+  float gmx, gmy, gmz;  // gravity * mass
+  float damp, kd, k0, k1, k2, rest_scale, eps;
+  uint32_t ntypes;
+};
+
+// One thread per vertex.  Spring forces use FMA and rsqrt (positions are
+// checked against the FP64 oracle by tolerance); the collision part is exact
+// order so the contact set is bit-exact for the pre-projection positions.
+template <bool DEBUG, bool COUNT>
+__global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
+                                                float* __restrict__ contact, const uint32_t* __restrict__ row,
+                                                const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
+                                                uint32_t n, uint64_t* __restrict__ frame,
+                                                float* __restrict__ dbg_pre, int32_t* __restrict__ dbg_hit,
+                                                uint8_t* __restrict__ dbg_in,
+                                                unsigned long long* __restrict__ ctr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) frame[0] += 1;
+  uint32_t visits = 0;
+  bool active = i < n;
+  if (active) {
+    const uint32_t r0raw = row[i], r1 = row[i + 1] & ROW_MASK;
+    const uint32_t r0 = r0raw & ROW_MASK;
+    const float* s = sin_ + 6 * (size_t)i;
+    const f3 p = mk(s[0], s[1], s[2]), q = mk(s[3], s[4], s[5]);
+    f3 np, nq, nc = mk(0.0f, 0.0f, 0.0f);
+    int32_t hit = -1;
+    bool inside = false;
+    if (r0raw & PIN_BIT) {
+      np = p;
+      nq = p;
+    } else {
+      const f3 vel = mk((p.x - q.x) * a.inv_dt, (p.y - q.y) * a.inv_dt, (p.z - q.z) * a.inv_dt);
+      f3 F = mk(fmaf(vel.x, a.damp, a.gmx), fmaf(vel.y, a.damp, a.gmy), fmaf(vel.z, a.damp, a.gmz));
+      f3 S = mk(0.f, 0.f, 0.f);   // all spring types, in merged-row order
+      for (uint32_t k = r0; k < r1; ++k) {
+        const uint2 r = rec[k];
+        const uint32_t j = r.x & END_MASK, t = r.x >> 30;
+        const float rest = __uint_as_float(r.y) * a.rest_scale;
+        const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
+        const float* sj = sin_ + 6 * (size_t)j;
+        const f3 p2 = mk(sj[0], sj[1], sj[2]), q2 = mk(sj[3], sj[4], sj[5]);
+        const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
+        const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
+                         fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
+        const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
+        const float rinv = rsqrtf(L2);
+        const float L = L2 * rinv;
+        const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
+        const float sc = fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv;
+        S.x = fmaf(dP.x, sc, S.x);
+        S.y = fmaf(dP.y, sc, S.y);
+        S.z = fmaf(dP.z, sc, S.z);
+      }
+      F = F + S;
+      const f3 c = ld3(contact + 3 * (size_t)i);
+      const float cf = fmaf(c.z, F.z, fmaf(c.y, F.y, c.x * F.x));
+      F = mk(fmaf(-c.x, cf, F.x), fmaf(-c.y, cf, F.y), fmaf(-c.z, cf, F.z));
+      // pos + pos - pos_old + acc*dt*dt (verlet.cu:155) evaluated as
+      // pos + ((pos - pos_old) + acc*dt^2): pos - pos_old is exact for close
+      // values, so the only rounding at |pos| is the last addition.
+      np = mk(p.x + fmaf(F.x * a.inv_mass, a.dt2, p.x - q.x), p.y + fmaf(F.y * a.inv_mass, a.dt2, p.y - q.y),
+              p.z + fmaf(F.z * a.inv_mass, a.dt2, p.z - q.z));
+      nq = p;
+      if (DEBUG) st3(dbg_pre + 3 * (size_t)i, np);
+      inside = collide_project(bvh, np, nc, hit, visits);
+      if (inside) nq = np;
+    }
+    if (DEBUG) {
+      if (r0raw & PIN_BIT) st3(dbg_pre + 3 * (size_t)i, np);
+      dbg_hit[i] = hit;
+      dbg_in[i] = inside ? 1 : 0;
+    }
+    float* o = sout + 6 * (size_t)i;
+    o[0] = np.x; o[1] = np.y; o[2] = np.z;
+    o[3] = nq.x; o[4] = nq.y; o[5] = nq.z;
+    st3(contact + 3 * (size_t)i, nc);
+  }
+  if (COUNT) {
+    // wave-level sum, one atomic per wave
+    unsigned long long v = visits, qn = active ? 1ull : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+      v += __shfl_down(v, off, 64);
+      qn += __shfl_down(qn, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(ctr, v);
+      atomicAdd(ctr + 1, qn);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float* __restrict__ pts, uint32_t n,
+                                                        int32_t* __restrict__ hit, uint8_t* __restrict__ ins,
+                                                        float* __restrict__ out_pos, float* __restrict__ out_n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  f3 p = ld3(pts + 3 * (size_t)i), nc;
+  int32_t h;
+  uint32_t visits = 0;
+  const bool in = collide_project(bvh, p, nc, h, visits);
+  hit[i] = h;
+  ins[i] = in ? 1 : 0;
+  st3(out_pos + 3 * (size_t)i, p);
+  st3(out_n + 3 * (size_t)i, nc);
+}
+
+__global__ void k_finite(const float* __restrict__ s, uint32_t n, uint32_t* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  bool ok = true;
+  for (int c = 0; c < 6; ++c) ok = ok && isfinite(s[6 * (size_t)i + c]);
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// compute_face_normal (verlet.cu:91-116)
+__global__ void k_face_normals(const float* __restrict__ s, const uint32_t* __restrict__ faces, uint32_t nf,
+                               float* __restrict__ fn) {
+  const uint32_t f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= nf) return;
+  const uint32_t* t = faces + 3 * (size_t)f;
+  const f3 a = ld3(s + 6 * (size_t)t[0]), b = ld3(s + 6 * (size_t)t[1]), c = ld3(s + 6 * (size_t)t[2]);
+  st3(fn + 3 * (size_t)f, normalize(cross(b - a, c - a)));
+}
+
+// update_vbo_pos + compute_vbo_normal (verlet.cu:119-127, 163-181)
+__global__ void k_vbo(const float* __restrict__ s, const uint32_t* __restrict__ vrow,
+                      const uint32_t* __restrict__ vcol, const float* __restrict__ fn, uint32_t n,
+                      float* __restrict__ vbo_pos, float* __restrict__ vbo_nrm, float* __restrict__ keep_n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float* x = s + 6 * (size_t)i;
+  reinterpret_cast<float4*>(vbo_pos)[i] = make_float4(x[0], x[1], x[2], 1.0f);
+  f3 nn = mk(0.0f, 0.0f, 0.0f);
+  for (uint32_t k = vrow[i]; k < vrow[i + 1]; ++k) nn = nn + ld3(fn + 3 * (size_t)vcol[k]);
+  nn = normalize(nn);
+  st3(vbo_nrm + 3 * (size_t)i, nn);
+  st3(keep_n + 3 * (size_t)i, nn);
+}
+
+// ------------------------------------------------------------------ Sim
+struct Sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, ntypes = 0;
+  cc_params prm{};
+  Body body;
+  DBuf<float> state[2];
+  DBuf<float> contact;
+  DBuf<uint32_t> row;
+  DBuf<uint2> rec;
+  std::vector<uint32_t> h_row;
+  int cur = 0;
+  DBuf<uint64_t> frame_dev;
+  uint64_t frame = 0;
+  bool debug = false, counting = false;
+  DBuf<float> dbg_pre;
+  DBuf<int32_t> dbg_hit;
+  DBuf<uint8_t> dbg_in;
+  DBuf<unsigned long long> ctr;
+  DBuf<uint32_t> flag;
+  // VBO
+  uint32_t nf = 0;
+  DBuf<uint32_t> faces, vrow, vcol;
+  DBuf<float> face_n, vnormals;
+  // graphs: one captured step per ping-pong parity
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+  ~Sim() {
+    drop_graphs();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void drop_graphs() {
+    for (auto& g : gexec) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+
+  StepArgs args() const {
+    StepArgs a;
+    a.inv_dt = 1.0f / prm.dt;
+    a.dt2 = prm.dt * prm.dt;
+    a.inv_mass = 1.0f / prm.mass;
+    a.gmx = prm.gravity[0] * prm.mass;
+    a.gmy = prm.gravity[1] * prm.mass;
+    a.gmz = prm.gravity[2] * prm.mass;
+    a.damp = prm.damp;
+    a.kd = prm.kd;
+    a.k0 = prm.k[0];
+    a.k1 = prm.k[1];
+    a.k2 = prm.k[2];
+    a.rest_scale = prm.rest_scale;
+    a.eps = prm.eps;
+    a.ntypes = ntypes;
+    return a;
+  }
+
+  void launch_verlet(hipStream_t s) {
+    const float* in = state[cur].p;
+    float* out = state[cur ^ 1].p;
+    BvhView v = body.view();
+    const unsigned g = grid_for(n, 256);
+    if (debug && counting)
+      k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
+                                             dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    else if (debug)
+      k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
+                                              dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    else if (counting)
+      k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
+                                              dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    else
+      k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
+                                               dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    CC_HIP(hipGetLastError());
+  }
+
+  // one frame: body update, then the fused step (reads state[cur])
+  void enqueue_step(hipStream_t s) {
+    body.frame_update(frame_dev.p, (double)prm.dt, s);
+    launch_verlet(s);
+  }
+
+  void step(int nsteps, hipStream_t s) {
+    if (!s) s = stream;
+    for (int it = 0; it < nsteps; ++it) {
+      hipGraphExec_t& g = gexec[cur];
+      if (!g) {
+        hipGraph_t graph;
+        CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_step(stream);
+        } catch (...) {
+          hipGraph_t junk;
+          (void)hipStreamEndCapture(stream, &junk);
+          if (junk) (void)hipGraphDestroy(junk);
+          throw;
+        }
+        CC_HIP(hipStreamEndCapture(stream, &graph));
+        hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          g = nullptr;
+          throw Error(CC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+      }
+      CC_HIP(hipGraphLaunch(g, s));
+      cur ^= 1;
+      ++frame;
+    }
+  }
+};
+
+}  // namespace cc
+
+using namespace cc;
+
+struct cc_sim {
+  Sim s;
+};
+
+static void merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec) {
+  const uint32_t n = c->n;
+  row.assign((size_t)n + 1, 0);
+  size_t total = 0;
+  for (uint32_t t = 0; t < c->ntypes; ++t) {
+    CC_REQUIRE(c->row[t] && (c->col[t] || c->row[t][n] == 0), "cloth: CSR arrays required for each type");
+    CC_REQUIRE(c->row[t][0] == 0, "cloth: CSR row must start at 0");
+    for (uint32_t i = 0; i < n; ++i) CC_REQUIRE(c->row[t][i] <= c->row[t][i + 1], "cloth: CSR row not ascending");
+    total += c->row[t][n];
+  }
+  CC_REQUIRE(total < ROW_MASK, "cloth: too many springs");
+  rec.resize(total);
+  size_t k = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    row[i] = (uint32_t)k;
+    for (uint32_t t = 0; t < c->ntypes; ++t)
+      for (uint32_t e = c->row[t][i]; e < c->row[t][i + 1]; ++e) {
+        const cc_spring& sp = c->col[t][e];
+        CC_REQUIRE(sp.end >= 0 && (uint32_t)sp.end < n, "cloth: spring end out of range");
+        uint32_t bits;
+        std::memcpy(&bits, &sp.original, 4);
+        rec[k++] = make_uint2((uint32_t)sp.end | (t << 30), bits);
+      }
+  }
+  row[n] = (uint32_t)k;
+}
+
+static void read_vec3_from(const float* v, uint32_t stride, uint32_t n, std::vector<float>& out) {
+  out.resize(3 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = v[(size_t)stride * i + c];
+}
+
+extern "C" {
+
+int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_params* params, cc_sim** out) {
+  return guard([&] {
+    CC_REQUIRE(cloth && params && out, "cc_sim_create: null argument");
+    CC_REQUIRE(cloth->n >= 1 && cloth->verts4, "cc_sim_create: empty cloth");
+    CC_REQUIRE(cloth->ntypes >= 1 && cloth->ntypes <= 3, "cc_sim_create: ntypes must be 1..3");
+    CC_REQUIRE(params->dt > 0.0f && params->mass > 0.0f, "cc_sim_create: dt and mass must be positive");
+    *out = nullptr;
+    int ndev = 0;
+    CC_HIP(hipGetDeviceCount(&ndev));
+    CC_REQUIRE(ndev > 0, "cc_sim_create: no HIP device");
+    std::unique_ptr<cc_sim> h(new cc_sim);
+    Sim& S = h->s;
+    CC_HIP(hipGetDevice(&S.device));
+    CC_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+    S.n = cloth->n;
+    S.ntypes = cloth->ntypes;
+    S.prm = *params;
+    hipStream_t s = S.stream;
+    std::vector<uint2> rec;
+    merge_springs(cloth, S.h_row, rec);
+    S.row.upload(S.h_row.data(), S.h_row.size(), s);
+    S.rec.upload(rec.data(), std::max<size_t>(rec.size(), 1), s);
+    // x_last = x_cur, contact = 0 (simulation.cu:65-85)
+    std::vector<float> x3;
+    read_vec3_from(cloth->verts4, 4, S.n, x3);
+    std::vector<float> packed(6 * (size_t)S.n);
+    for (uint32_t i = 0; i < S.n; ++i)
+      for (int c = 0; c < 3; ++c) {
+        packed[6 * (size_t)i + c] = x3[3 * (size_t)i + c];
+        packed[6 * (size_t)i + 3 + c] = x3[3 * (size_t)i + c];
+      }
+    S.state[0].upload(packed.data(), packed.size(), s);
+    S.state[1].alloc(packed.size());
+    S.state[1].zero(s);
+    S.contact.alloc(3 * (size_t)S.n);
+    S.contact.zero(s);
+    S.frame_dev.alloc(1);
+    S.frame_dev.zero(s);
+    S.ctr.alloc(2);
+    S.ctr.zero(s);
+    S.flag.alloc(1);
+    if (bd && bd->mode != CC_BODY_NONE) {
+      CC_REQUIRE(bd->verts && bd->tris && (bd->stride == 3 || bd->stride == 4), "cc_sim_create: bad body");
+      std::vector<float> bv;
+      read_vec3_from(bd->verts, bd->stride, bd->nv, bv);
+      if (bd->extend) {
+        CC_REQUIRE(bd->normals3, "cc_sim_create: body normals required for extend");
+        int rc = cc_mesh_vertex_extend(bv.data(), 3, bd->nv, bd->tris, bd->nf, nullptr, bd->normals3,
+                                       params->body_extend);
+        if (rc) throw Error(rc, cc_last_error());
+      }
+      MotionDesc md;
+      md.amp = bd->amp;
+      md.freq = bd->freq;
+      md.rot_amp = bd->rot_amp;
+      md.breath_amp = bd->breath_amp;
+      md.breath_freq = bd->breath_freq;
+      for (int c = 0; c < 3; ++c) {
+        md.dir[c] = bd->dir[c];
+        md.pivot[c] = bd->pivot[c];
+      }
+      S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, md, s);
+      // the topology is built once from the frame-0 pose
+      S.body.deform(S.frame_dev.p, (double)params->dt, s);
+      S.body.build(s);
+    }
+    CC_HIP(hipStreamSynchronize(s));
+    *out = h.release();
+  });
+}
+
+int cc_sim_destroy(cc_sim* sim) {
+  return guard([&] {
+    if (!sim) return;
+    (void)hipStreamSynchronize(sim->s.stream);
+    delete sim;
+  });
+}
+
+int cc_sim_step(cc_sim* sim, int nsteps, void* stream) {
+  return guard([&] {
+    CC_REQUIRE(sim && nsteps >= 0, "cc_sim_step: bad argument");
+    sim->s.step(nsteps, (hipStream_t)stream);
+  });
+}
+
+int cc_sim_synchronize(cc_sim* sim) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_synchronize: null");
+    CC_HIP(hipStreamSynchronize(sim->s.stream));
+    CC_HIP(hipDeviceSynchronize());
+  });
+}
+
+int cc_sim_get_state(cc_sim* sim, float* x_cur, float* x_prev, float* contact_n) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_get_state: null");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    std::vector<float> packed(6 * (size_t)S.n);
+    CC_HIP(hipMemcpy(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < S.n; ++i)
+      for (int c = 0; c < 3; ++c) {
+        if (x_cur) x_cur[3 * (size_t)i + c] = packed[6 * (size_t)i + c];
+        if (x_prev) x_prev[3 * (size_t)i + c] = packed[6 * (size_t)i + 3 + c];
+      }
+    if (contact_n) CC_HIP(hipMemcpy(contact_n, S.contact.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int cc_sim_set_state(cc_sim* sim, const float* x_cur, const float* x_prev, const float* contact_n) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_set_state: null");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    std::vector<float> packed(6 * (size_t)S.n);
+    CC_HIP(hipMemcpy(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < S.n; ++i)
+      for (int c = 0; c < 3; ++c) {
+        if (x_cur) packed[6 * (size_t)i + c] = x_cur[3 * (size_t)i + c];
+        if (x_prev) packed[6 * (size_t)i + 3 + c] = x_prev[3 * (size_t)i + c];
+      }
+    CC_HIP(hipMemcpy(S.state[S.cur].p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
+    if (contact_n) CC_HIP(hipMemcpy(S.contact.p, contact_n, 3 * (size_t)S.n * 4, hipMemcpyHostToDevice));
+  });
+}
+
+int cc_sim_frame(cc_sim* sim, uint64_t* frame) {
+  return guard([&] {
+    CC_REQUIRE(sim && frame, "cc_sim_frame: null");
+    *frame = sim->s.frame;
+  });
+}
+
+int cc_sim_set_frame(cc_sim* sim, uint64_t frame) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_set_frame: null");
+    CC_HIP(hipDeviceSynchronize());
+    CC_HIP(hipMemcpy(sim->s.frame_dev.p, &frame, 8, hipMemcpyHostToDevice));
+    sim->s.frame = frame;
+  });
+}
+
+int cc_sim_update_vertex(cc_sim* sim, uint32_t idx, const float xyz[3]) {
+  return guard([&] {
+    CC_REQUIRE(sim && xyz && idx < sim->s.n, "cc_sim_update_vertex: bad argument");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    float rec6[6] = {xyz[0], xyz[1], xyz[2], xyz[0], xyz[1], xyz[2]};
+    CC_HIP(hipMemcpy(S.state[S.cur].p + 6 * (size_t)idx, rec6, sizeof(rec6), hipMemcpyHostToDevice));
+  });
+}
+
+int cc_sim_set_pins(cc_sim* sim, const uint32_t* idx, uint32_t k) {
+  return guard([&] {
+    CC_REQUIRE(sim && (idx || k == 0), "cc_sim_set_pins: bad argument");
+    Sim& S = sim->s;
+    for (uint32_t q = 0; q < k; ++q) CC_REQUIRE(idx[q] < S.n, "cc_sim_set_pins: index out of range");
+    CC_HIP(hipDeviceSynchronize());
+    for (uint32_t i = 0; i < S.n; ++i) S.h_row[i] &= ROW_MASK;
+    std::vector<float> rec6(6);
+    for (uint32_t q = 0; q < k; ++q) {
+      S.h_row[idx[q]] |= PIN_BIT;
+      float* dst = S.state[S.cur].p + 6 * (size_t)idx[q];
+      CC_HIP(hipMemcpy(rec6.data(), dst, 24, hipMemcpyDeviceToHost));
+      for (int c = 0; c < 3; ++c) rec6[3 + c] = rec6[c];
+      CC_HIP(hipMemcpy(dst, rec6.data(), 24, hipMemcpyHostToDevice));
+    }
+    CC_HIP(hipMemcpy(S.row.p, S.h_row.data(), S.h_row.size() * 4, hipMemcpyHostToDevice));
+  });
+}
+
+int cc_sim_set_params(cc_sim* sim, const cc_params* p) {
+  return guard([&] {
+    CC_REQUIRE(sim && p && p->dt > 0.0f && p->mass > 0.0f, "cc_sim_set_params: bad argument");
+    CC_HIP(hipDeviceSynchronize());
+    sim->s.prm = *p;
+    sim->s.body.depth = p->box_depth;
+    sim->s.drop_graphs();
+  });
+}
+
+int cc_sim_set_debug(cc_sim* sim, int enable) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_set_debug: null");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    S.debug = enable != 0;
+    if (S.debug) {
+      S.dbg_pre.alloc(3 * (size_t)S.n);
+      S.dbg_hit.alloc(S.n);
+      S.dbg_in.alloc(S.n);
+    }
+    S.drop_graphs();
+  });
+}
+
+int cc_sim_get_debug(cc_sim* sim, float* pre3, int32_t* hit, uint8_t* inside) {
+  return guard([&] {
+    CC_REQUIRE(sim && sim->s.debug, "cc_sim_get_debug: debug outputs not enabled");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    if (pre3) CC_HIP(hipMemcpy(pre3, S.dbg_pre.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost));
+    if (hit) CC_HIP(hipMemcpy(hit, S.dbg_hit.p, (size_t)S.n * 4, hipMemcpyDeviceToHost));
+    if (inside) CC_HIP(hipMemcpy(inside, S.dbg_in.p, S.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int cc_sim_set_counting(cc_sim* sim, int enable) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_set_counting: null");
+    CC_HIP(hipDeviceSynchronize());
+    sim->s.counting = enable != 0;
+    CC_HIP(hipMemset(sim->s.ctr.p, 0, 16));
+    sim->s.drop_graphs();
+  });
+}
+
+int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_get_counters: null");
+    CC_HIP(hipDeviceSynchronize());
+    unsigned long long c[2];
+    CC_HIP(hipMemcpy(c, sim->s.ctr.p, 16, hipMemcpyDeviceToHost));
+    if (nodes_visited) *nodes_visited = c[0];
+    if (queries) *queries = c[1];
+  });
+}
+
+int cc_sim_check_finite(cc_sim* sim) {
+  int rc = guard([&] {
+    CC_REQUIRE(sim, "cc_sim_check_finite: null");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    CC_HIP(hipMemset(S.flag.p, 0, 4));
+    k_finite<<<grid_for(S.n, 256), 256, 0, S.stream>>>(S.state[S.cur].p, S.n, S.flag.p);
+    CC_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    CC_HIP(hipMemcpyAsync(&bad, S.flag.p, 4, hipMemcpyDeviceToHost, S.stream));
+    CC_HIP(hipStreamSynchronize(S.stream));
+    if (bad) throw Error(CC_ERR_NONFINITE, "cc_sim_check_finite: non-finite position found");
+  });
+  return rc;
+}
+
+int cc_body_set_vertices(cc_sim* sim, const float* verts3) {
+  return guard([&] {
+    CC_REQUIRE(sim && verts3, "cc_body_set_vertices: bad argument");
+    Body& B = sim->s.body;
+    CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_set_vertices: no body");
+    CC_HIP(hipDeviceSynchronize());
+    CC_HIP(hipMemcpy(B.rest.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
+    CC_HIP(hipMemcpy(B.verts.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
+    B.build(sim->s.stream);
+    CC_HIP(hipStreamSynchronize(sim->s.stream));
+  });
+}
+
+int cc_body_get_vertices(cc_sim* sim, float* verts3) {
+  return guard([&] {
+    CC_REQUIRE(sim && verts3, "cc_body_get_vertices: bad argument");
+    Body& B = sim->s.body;
+    CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_get_vertices: no body");
+    CC_HIP(hipDeviceSynchronize());
+    CC_HIP(hipMemcpy(verts3, B.verts.p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_body_get_motion: null");
+    Body& B = sim->s.body;
+    CC_REQUIRE(B.ncaps > 0, "cc_body_get_motion: body has no motion");
+    CC_HIP(hipDeviceSynchronize());
+    std::vector<float> m(12 + B.ncaps);
+    CC_HIP(hipMemcpy(m.data(), B.motion_out.p, m.size() * 4, hipMemcpyDeviceToHost));
+    if (R) std::memcpy(R, m.data(), 36);
+    if (T) std::memcpy(T, m.data() + 9, 12);
+    if (breath) std::memcpy(breath, m.data() + 12, 4 * (size_t)B.ncaps);
+  });
+}
+
+int cc_bvh_rebuild(cc_sim* sim) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_bvh_rebuild: null");
+    CC_HIP(hipDeviceSynchronize());
+    sim->s.body.build(sim->s.stream);
+    CC_HIP(hipStreamSynchronize(sim->s.stream));
+  });
+}
+
+int cc_bvh_refit(cc_sim* sim) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_bvh_refit: null");
+    CC_HIP(hipDeviceSynchronize());
+    sim->s.body.refit(sim->s.stream);
+    CC_HIP(hipStreamSynchronize(sim->s.stream));
+  });
+}
+
+int cc_bvh_size(cc_sim* sim, uint32_t* leaves) {
+  return guard([&] {
+    CC_REQUIRE(sim && leaves, "cc_bvh_size: bad argument");
+    *leaves = sim->s.body.mode == CC_BODY_NONE ? 0 : sim->s.body.m;
+  });
+}
+
+int cc_bvh_export(cc_sim* sim, uint64_t* keys, uint32_t* order, int32_t* childA, int32_t* childB,
+                  int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_bvh_export: null");
+    CC_HIP(hipDeviceSynchronize());
+    sim->s.body.export_tree(keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
+                            sim->s.stream);
+  });
+}
+
+int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, uint8_t* inside, float* out_pos,
+                      float* out_n) {
+  return guard([&] {
+    CC_REQUIRE(sim && pts3 && hit && inside && out_pos && out_n, "cc_collide_points: null argument");
+    if (n == 0) return;
+    Sim& S = sim->s;
+    hipStream_t s = S.stream;
+    CC_HIP(hipDeviceSynchronize());
+    DBuf<float> p, op, on;
+    DBuf<int32_t> h;
+    DBuf<uint8_t> in;
+    p.upload(pts3, 3 * (size_t)n, s);
+    op.alloc(3 * (size_t)n);
+    on.alloc(3 * (size_t)n);
+    h.alloc(n);
+    in.alloc(n);
+    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(), p.p, n, h.p, in.p, op.p, on.p);
+    CC_HIP(hipGetLastError());
+    CC_HIP(hipMemcpyAsync(hit, h.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
+    CC_HIP(hipMemcpyAsync(out_pos, op.p, 12 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CC_HIP(hipMemcpyAsync(out_n, on.p, 12 * (size_t)n, hipMemcpyDeviceToHost, s));
+    CC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// Simulator::get_vertex_adjface (simulation.cu:152-180)
+int cc_sim_set_faces(cc_sim* sim, const uint32_t* f3i, uint32_t nf) {
+  return guard([&] {
+    CC_REQUIRE(sim && f3i && nf > 0, "cc_sim_set_faces: bad argument");
+    Sim& S = sim->s;
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i) CC_REQUIRE(f3i[i] < S.n, "cc_sim_set_faces: index out of range");
+    std::vector<std::vector<uint32_t>> adj(S.n);
+    for (uint32_t f = 0; f < nf; ++f)
+      for (int j = 0; j < 3; ++j) adj[f3i[3 * (size_t)f + j]].push_back(f);
+    std::vector<uint32_t> r, c;
+    r.reserve(S.n + 1);
+    for (auto& a : adj) {
+      r.push_back((uint32_t)c.size());
+      c.insert(c.end(), a.begin(), a.end());
+    }
+    r.push_back((uint32_t)c.size());
+    CC_HIP(hipDeviceSynchronize());
+    S.nf = nf;
+    S.faces.upload(f3i, 3 * (size_t)nf, S.stream);
+    S.vrow.upload(r.data(), r.size(), S.stream);
+    S.vcol.upload(c.data(), c.size(), S.stream);
+    S.face_n.alloc(3 * (size_t)nf);
+    S.vnormals.alloc(3 * (size_t)S.n);
+    CC_HIP(hipStreamSynchronize(S.stream));
+  });
+}
+
+int cc_sim_write_vbo(cc_sim* sim, void* dst_dev, size_t bytes, uint32_t n_tex, int lag) {
+  return guard([&] {
+    CC_REQUIRE(sim && dst_dev, "cc_sim_write_vbo: null");
+    Sim& S = sim->s;
+    CC_REQUIRE(S.nf > 0, "cc_sim_write_vbo: call cc_sim_set_faces first");
+    const size_t need = 16 * (size_t)S.n + 8 * (size_t)n_tex + 12 * (size_t)S.n;
+    CC_REQUIRE(bytes >= need, "cc_sim_write_vbo: buffer too small");
+    float* base = (float*)dst_dev;
+    float* nrm = base + 4 * (size_t)S.n + 2 * (size_t)n_tex;   // simulation.cu:210
+    // positions are x_cur_out of the last step == state[cur]; the reference
+    // computes face normals from x_cur_in (the step before), kept by lag=1
+    const float* pos_state = S.state[S.cur].p;
+    const float* fn_state = lag ? S.state[S.cur ^ 1].p : pos_state;
+    k_face_normals<<<grid_for(S.nf, 256), 256, 0, S.stream>>>(fn_state, S.faces.p, S.nf, S.face_n.p);
+    k_vbo<<<grid_for(S.n, 256), 256, 0, S.stream>>>(pos_state, S.vrow.p, S.vcol.p, S.face_n.p, S.n, base, nrm,
+                                                    S.vnormals.p);
+    CC_HIP(hipGetLastError());
+    CC_HIP(hipStreamSynchronize(S.stream));
+  });
+}
+
+int cc_sim_get_normals(cc_sim* sim, float* normals3) {
+  return guard([&] {
+    CC_REQUIRE(sim && normals3 && sim->s.nf > 0, "cc_sim_get_normals: bad argument");
+    CC_HIP(hipDeviceSynchronize());
+    CC_HIP(hipMemcpy(normals3, sim->s.vnormals.p, 12 * (size_t)sim->s.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
+  return guard([&] {
+    CC_REQUIRE(sim && out && nsteps > 0, "cc_sim_time_kernels: bad argument");
+    Sim& S = sim->s;
+    hipStream_t s = S.stream;
+    CC_HIP(hipDeviceSynchronize());
+    std::memset(out, 0, sizeof(*out));
+    std::vector<hipEvent_t> ev(5);
+    for (auto& e : ev) CC_HIP(hipEventCreate(&e));
+    double t_body = 0, t_upd = 0, t_ver = 0;
+    for (int it = 0; it < nsteps; ++it) {
+      CC_HIP(hipEventRecord(ev[0], s));
+      if (S.body.mode == CC_BODY_REFIT || S.body.mode == CC_BODY_REBUILD) S.body.deform(S.frame_dev.p, S.prm.dt, s);
+      CC_HIP(hipEventRecord(ev[1], s));
+      if (S.body.mode == CC_BODY_REFIT) S.body.refit(s);
+      else if (S.body.mode == CC_BODY_REBUILD) S.body.build(s);
+      CC_HIP(hipEventRecord(ev[2], s));
+      S.launch_verlet(s);
+      CC_HIP(hipEventRecord(ev[3], s));
+      CC_HIP(hipEventSynchronize(ev[3]));
+      float a, b, c;
+      CC_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+      CC_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+      CC_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+      t_body += a;
+      t_upd += b;
+      t_ver += c;
+      S.cur ^= 1;
+      ++S.frame;
+    }
+    // whole steps from the captured graphs
+    CC_HIP(hipEventRecord(ev[0], s));
+    S.step(nsteps, s);
+    CC_HIP(hipEventRecord(ev[4], s));
+    CC_HIP(hipEventSynchronize(ev[4]));
+    float w;
+    CC_HIP(hipEventElapsedTime(&w, ev[0], ev[4]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    out->steps = (uint32_t)nsteps;
+    out->body_us = 1e3 * t_body / nsteps;
+    if (S.body.mode == CC_BODY_REFIT) out->refit_us = 1e3 * t_upd / nsteps;
+    else out->build_us = 1e3 * t_upd / nsteps;
+    out->verlet_us = 1e3 * t_ver / nsteps;
+    out->step_us = 1e3 * w / nsteps;
+  });
+}
+
+}  // extern "C"
