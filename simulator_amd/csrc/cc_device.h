@@ -18,7 +18,11 @@ namespace cc {
 __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits) {
   if (b.m == 0) return -1;
   if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
-  int32_t stack[64];
+  // Short stack: the top four entries live in registers (static shifts, no
+  // indexed private memory); older entries spill to a private array that is
+  // touched only by paths deeper than four deferred nodes.
+  int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int32_t spill[60];
   int sp = 0;
   int32_t node = 0;
   // A well-formed tree visits each internal node at most once; the bound
@@ -38,16 +42,26 @@ __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits
     if (oR && cb < 0) return ~cb;
     const bool tL = oL && ca >= 0, tR = oR && cb >= 0;
     if (tL) {
-      if (tR) {
-        if (sp == 64) return -1;   // unreachable for depth < 64
-        stack[sp++] = cb;
+      if (tR) {                           // push cb
+        if (sp == 64) return -1;          // unreachable for depth < 64
+        if (sp >= 4) spill[sp - 4] = s3;
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = cb;
+        ++sp;
       }
       node = ca;
     } else if (tR) {
       node = cb;
-    } else {
+    } else {                              // pop
       if (sp == 0) return -1;
-      node = stack[--sp];
+      node = s0;
+      s0 = s1;
+      s1 = s2;
+      s2 = s3;
+      --sp;
+      if (sp >= 4) s3 = spill[sp - 4];
     }
   }
   return -1;

@@ -22,11 +22,15 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& msg);
 
+// On failure the runtime's sticky last-error is cleared as well, so a later
+// hipGetLastError() after a launch reports only that launch.
 #define CC_HIP(expr)                                                                  \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
+    if (e_ != hipSuccess) {                                                           \
+      (void)hipGetLastError();                                                        \
       throw ::cc::Error(CC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    }                                                                                 \
   } while (0)
 
 #define CC_REQUIRE(cond, msg)                                       \
@@ -78,7 +82,7 @@ struct DBuf {
   }
   void upload(const T* h, size_t count, hipStream_t s) {
     alloc(count);
-    if (count) CC_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    if (count && h) CC_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
   }
   void zero(hipStream_t s) {
     if (n) CC_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));

@@ -1,6 +1,7 @@
 // lbvh.hip -- body deformation and the Karras LBVH on gfx950: expanded
 // triangle boxes, 30-bit Morton codes, LSD radix sort (LDS histogram +
-// wavefront-ballot ranking), clz64 radix-tree build and atomic refit.
+// wavefront-ballot ranking) and the clz64 radix-tree build; the box refit
+// lives in refit.hip.
 //
 // Reference path: BVHAccel (bvh.cu:154-297) computes boxes and codes on the
 // GPU, sorts on the host (Utilities.h:22-67) and builds the tree with
@@ -15,63 +16,45 @@
 namespace cc {
 
 // ------------------------------------------------------------ deformation
-// One block computes the frame's float parameters in LDS (double math, one
-// rounding to float), then every thread moves one vertex: radial breathing
-// about the closest point of its capsule axis, then R * Y + T.
-struct MotionArgs {
-  double amp, freq, dx, dy, dz, rot_amp, px, py, pz, breath_amp, breath_freq, dt;
-};
-
+// k_motion computes the frame's parameters once (one block) from the body's
+// frame counter and advances the counter; then every k_deform thread moves
+// one vertex: radial breathing about the closest point of its capsule axis,
+// then R * Y + T.
 constexpr int MAX_CAPS = 64;
+
+__global__ __launch_bounds__(64) void k_motion(MotionArgs a) {
+  const uint64_t f = *a.frame;
+  __syncthreads();
+  compute_motion(a, f, threadIdx.x, blockDim.x);
+  if (threadIdx.x == 0) *a.frame = f + 1;
+}
 
 __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, float* __restrict__ out,
                                                 uint32_t nv, uint32_t ncaps,
                                                 const uint32_t* __restrict__ vstart,
                                                 const float* __restrict__ p0, const float* __restrict__ p1,
-                                                const float* __restrict__ phase, MotionArgs a,
-                                                const uint64_t* __restrict__ frame_dev,
-                                                float* __restrict__ motion_out) {
-  __shared__ float sR[9], sT[3], sB[MAX_CAPS], sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
+                                                const float* __restrict__ par) {
+  __shared__ float sP[12 + MAX_CAPS], sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
   __shared__ uint32_t sV[MAX_CAPS + 1];
   const uint32_t tid = threadIdx.x;
-  const double PI2 = 6.283185307179586476925;
-  const double t = (double)(*frame_dev) * a.dt;
-  if (tid == 0) {
-    const double w = sin(PI2 * a.freq * t);
-    const double th = a.rot_amp * w, c = cos(th), s = sin(th);
-    // rotation about +y: [[c,0,s],[0,1,0],[-s,0,c]]
-    const double R[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
-    const double P[3] = {a.px, a.py, a.pz};
-    const double D[3] = {a.dx, a.dy, a.dz};
-    for (int r = 0; r < 3; ++r) {
-      double rp = R[3 * r] * P[0] + R[3 * r + 1] * P[1] + R[3 * r + 2] * P[2];
-      sT[r] = (float)(P[r] - rp + a.amp * w * D[r]);
-    }
-    for (int q = 0; q < 9; ++q) sR[q] = (float)R[q];
-  }
-  for (uint32_t k = tid; k < ncaps; k += blockDim.x) {
-    sB[k] = (float)(1.0 + a.breath_amp * sin(PI2 * a.breath_freq * t + (double)phase[k]));
-    for (int q = 0; q < 3; ++q) {
-      sP0[3 * k + q] = p0[3 * k + q];
-      sP1[3 * k + q] = p1[3 * k + q];
-    }
+  for (uint32_t k = tid; k < 12 + ncaps; k += blockDim.x) sP[k] = par[k];
+  for (uint32_t k = tid; k < 3 * ncaps; k += blockDim.x) {
+    sP0[k] = p0[k];
+    sP1[k] = p1[k];
   }
   for (uint32_t k = tid; k <= ncaps; k += blockDim.x) sV[k] = vstart[k];
   __syncthreads();
-  if (blockIdx.x == 0) {
-    if (tid < 9) motion_out[tid] = sR[tid];
-    if (tid < 3) motion_out[9 + tid] = sT[tid];
-    for (uint32_t k = tid; k < ncaps; k += blockDim.x) motion_out[12 + k] = sB[k];
-  }
   const uint32_t v = blockIdx.x * blockDim.x + tid;
   if (v >= nv) return;
   // capsule of v: last k with vstart[k] <= v
   uint32_t lo = 0, hi = ncaps - 1;
   while (lo < hi) {
-    uint32_t mid = (lo + hi + 1) >> 1;
+    const uint32_t mid = (lo + hi + 1) >> 1;
     if (sV[mid] <= v) lo = mid; else hi = mid - 1;
   }
   const uint32_t k = lo;
+  const float* R = sP;
+  const float* T = sP + 9;
   const f3 X = ld3(rest + 3 * (size_t)v);
   const f3 P0 = ld3(sP0 + 3 * k), P1 = ld3(sP1 + 3 * k);
   const f3 d = P1 - P0;
@@ -80,11 +63,11 @@ __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, 
   if (dd > 0.0f) tt = dot(X - P0, d) / dd;
   tt = fminf(fmaxf(tt, 0.0f), 1.0f);
   const f3 A = P0 + d * tt;
-  const f3 Y = A + (X - A) * sB[k];
+  const f3 Y = A + (X - A) * sP[12 + k];
   float* o = out + 3 * (size_t)v;
-  o[0] = ((sR[0] * Y.x + sR[1] * Y.y) + sR[2] * Y.z) + sT[0];
-  o[1] = ((sR[3] * Y.x + sR[4] * Y.y) + sR[5] * Y.z) + sT[1];
-  o[2] = ((sR[6] * Y.x + sR[7] * Y.y) + sR[8] * Y.z) + sT[2];
+  o[0] = ((R[0] * Y.x + R[1] * Y.y) + R[2] * Y.z) + T[0];
+  o[1] = ((R[3] * Y.x + R[4] * Y.y) + R[5] * Y.z) + T[1];
+  o[2] = ((R[6] * Y.x + R[7] * Y.y) + R[8] * Y.z) + T[2];
 }
 
 // ------------------------------------------------------------ boxes + codes
@@ -277,7 +260,8 @@ __global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict_
 // processInternalNode / determineRange / findSplit (cudaBRT.cu:9-161) with
 // delta = clz64 of 64-bit keys (-1 outside [0, m)).
 __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ keys, int32_t m,
-                                                Node* __restrict__ nodes, int32_t* __restrict__ leaf_parent) {
+                                                Node* __restrict__ nodes, int32_t* __restrict__ leaf_parent,
+                                                uint32_t* __restrict__ top_count) {
   const int32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m - 1) return;
   const uint64_t ki = keys[i];
@@ -309,62 +293,9 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ key
   nodes[i].other = j;
   if (ca < 0) leaf_parent[split] = i; else nodes[split].parent = i;
   if (cb < 0) leaf_parent[split + 1] = i; else nodes[split + 1].parent = i;
-  if (i == 0) nodes[0].parent = -1;
-}
-
-// ------------------------------------------------------------ refit
-// calculateBoudingBox (cudaBRT.cu:166-206): one thread per leaf computes the
-// leaf box, stores it in its parent's child slot and arrives at the parent;
-// the second arrival unites both child slots (sentinel U A U B) and climbs.
-// The arrival is an agent-scope acq_rel atomic, so the sibling's slot store
-// is visible (the reference has no fence).  The second arrival resets the
-// counter for the next refit.
-__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ verts,
-                                               const uint4* __restrict__ leaf_tri,
-                                               const int32_t* __restrict__ leaf_parent, Node* nodes,
-                                               uint32_t* counters, uint32_t m, float depth,
-                                               float* __restrict__ root_out, float* __restrict__ leaf0_box) {
-  const uint32_t l = blockIdx.x * 256 + threadIdx.x;
-  if (l >= m) return;
-  const uint4 t = leaf_tri[l];
-  float mn[3], mx[3];
-  prim_box(ld3(verts + 3 * (size_t)t.x), ld3(verts + 3 * (size_t)t.y), ld3(verts + 3 * (size_t)t.z), depth,
-           mn, mx);
-  if (m == 1) {
-    float smn[3], smx[3];
-    box_sentinel(smn, smx);
-    for (int c = 0; c < 3; ++c) {
-      leaf0_box[c] = mn[c];
-      leaf0_box[3 + c] = mx[c];
-      root_out[c] = fminf(smn[c], mn[c]);
-      root_out[3 + c] = fmaxf(smx[c], mx[c]);
-    }
-    return;
-  }
-  int32_t child = ~(int32_t)l;
-  int32_t x = leaf_parent[l];
-  while (true) {
-    Node* nd = nodes + x;
-    float* slot = (nd->childA == child) ? nd->a_min : nd->b_min;   // min[3] then max[3]
-    slot[0] = mn[0]; slot[1] = mn[1]; slot[2] = mn[2];
-    slot[3] = mx[0]; slot[4] = mx[1]; slot[5] = mx[2];
-    const uint32_t old = __hip_atomic_fetch_add(counters + x, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    counters[x] = 0;
-    box_sentinel(mn, mx);
-    for (int c = 0; c < 3; ++c) {
-      mn[c] = fminf(fminf(mn[c], nd->a_min[c]), nd->b_min[c]);
-      mx[c] = fmaxf(fmaxf(mx[c], nd->a_max[c]), nd->b_max[c]);
-    }
-    child = x;
-    x = nd->parent;
-    if (x < 0) {
-      for (int c = 0; c < 3; ++c) {
-        root_out[c] = mn[c];
-        root_out[3 + c] = mx[c];
-      }
-      return;
-    }
+  if (i == 0) {
+    nodes[0].parent = -1;
+    *top_count = 0;      // the refit top-tree list is rebuilt after the topology
   }
 }
 
@@ -381,10 +312,13 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   CC_REQUIRE(v3 && t3 && nv > 0, "body: vertices and triangles required");
   CC_REQUIRE(m < (1u << 30), "body: too many triangles");
   for (size_t i = 0; i < 3 * (size_t)m; ++i) CC_REQUIRE(t3[i] < nv, "body: triangle index out of range");
+  const int nbuf = moving() ? 2 : 1;
   rest.upload(v3, 3 * (size_t)nv, s);
-  verts.upload(v3, 3 * (size_t)nv, s);
+  for (int b = 0; b < nbuf; ++b) verts[b].upload(v3, 3 * (size_t)nv, s);
   tris.upload(t3, 3 * (size_t)m, s);
-  if (mode == CC_BODY_REFIT || mode == CC_BODY_REBUILD) {
+  frame_ctr.alloc(1);
+  frame_ctr.zero(s);
+  if (moving()) {
     CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
     CC_REQUIRE(caps->vstart[0] == 0 && caps->vstart[caps->ncaps] == nv, "body: capsule ranges must cover vertices");
     for (uint32_t k = 0; k < caps->ncaps; ++k)
@@ -394,8 +328,8 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
     cap_p0.upload(caps->p0, 3 * (size_t)ncaps, s);
     cap_p1.upload(caps->p1, 3 * (size_t)ncaps, s);
     cap_phase.upload(caps->phase, ncaps, s);
-    motion_out.alloc(12 + ncaps);
-    motion_out.zero(s);
+    motion_cur.alloc(12 + ncaps);
+    motion_cur.zero(s);
   }
   nblk_box = grid_for(m, BOX_THREADS);
   boxes.alloc(6 * (size_t)m);
@@ -406,32 +340,77 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   }
   hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
   keys.alloc(m);
-  leaf_tri.alloc(m);
   leaf_parent.alloc(m);
-  nodes.alloc(std::max<uint32_t>(m, 2) - 1);
-  counters.alloc(std::max<uint32_t>(m, 2) - 1);
-  counters.zero(s);
-  nodes.zero(s);
-  root.alloc(6);
-  leaf0_box.alloc(6);
-  leaf0_box.zero(s);
+  const size_t mi = std::max<uint32_t>(m, 2) - 1;
+  for (int b = 0; b < nbuf; ++b) {
+    leaf_tri[b].alloc(m);
+    nodes[b].alloc(mi);
+    nodes[b].zero(s);
+    root[b].alloc(6);
+    leaf0_box[b].alloc(6);
+    leaf0_box[b].zero(s);
+  }
+  top_list.alloc(mi);
+  top_pos.alloc(mi);
+  top_count.alloc(1);
+  top_count.zero(s);
+  top_links.alloc(mi);
+  top_child.alloc(12 * mi);
+  top_scratch.alloc(top_scratch_floats(m));
+  top_h.alloc(mi);
+  top_hmax.alloc(1);
+  top_hmax.zero(s);
+  top_lev.alloc(66);
+  top_lev.zero(s);
+  nflag.alloc(mi);
+  nflag.zero(s);
+  lflag.alloc(std::max<uint32_t>(m, 1));
+  lflag.zero(s);
+  chunk_h.alloc(grid_for(std::max<uint32_t>(m, 1), 2048));
+  chunk_h.zero(s);
 }
 
-void Body::deform(const uint64_t* frame_dev, double dt, hipStream_t s) {
+MotionArgs Body::motion_args() {
+  MotionArgs a;
+  a.amp = motion.amp;
+  a.freq = motion.freq;
+  a.dx = motion.dir[0];
+  a.dy = motion.dir[1];
+  a.dz = motion.dir[2];
+  a.rot_amp = motion.rot_amp;
+  a.px = motion.pivot[0];
+  a.py = motion.pivot[1];
+  a.pz = motion.pivot[2];
+  a.breath_amp = motion.breath_amp;
+  a.breath_freq = motion.breath_freq;
+  a.dt = dt;
+  a.phase = cap_phase.p;
+  a.out = motion_cur.p;
+  a.frame = frame_ctr.p;
+  a.ncaps = ncaps;
+  return a;
+}
+
+void Body::set_frame(uint64_t frame, hipStream_t s) {
+  if (mode == CC_BODY_NONE) return;
+  CC_HIP(hipMemcpyAsync(frame_ctr.p, &frame, sizeof(frame), hipMemcpyHostToDevice, s));
+  CC_HIP(hipStreamSynchronize(s));
+}
+
+void Body::deform(int b, hipStream_t s) {
   if (nv == 0 || ncaps == 0) return;
-  MotionArgs a{motion.amp,      motion.freq,      motion.dir[0],    motion.dir[1],
-               motion.dir[2],   motion.rot_amp,   motion.pivot[0],  motion.pivot[1],
-               motion.pivot[2], motion.breath_amp, motion.breath_freq, dt};
-  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, verts.p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
-                                               cap_phase.p, a, frame_dev, motion_out.p);
+  k_motion<<<1, 64, 0, s>>>(motion_args());
+  CC_HIP(hipGetLastError());
+  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, verts[b].p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
+                                               motion_cur.p);
   CC_HIP(hipGetLastError());
 }
 
-void Body::build(hipStream_t s) {
+void Body::build(int b, hipStream_t s) {
   if (mode == CC_BODY_NONE || m == 0) return;
-  k_prim_boxes<<<nblk_box, BOX_THREADS, 0, s>>>(verts.p, tris.p, m, depth, boxes.p, partial.p);
+  k_prim_boxes<<<nblk_box, BOX_THREADS, 0, s>>>(verts[b].p, tris.p, m, depth, boxes.p, partial.p);
   CC_HIP(hipGetLastError());
-  k_morton<<<grid_for(m, 256), 256, 0, s>>>(boxes.p, m, partial.p, nblk_box, code[0].p, idx[0].p, root.p);
+  k_morton<<<grid_for(m, 256), 256, 0, s>>>(boxes.p, m, partial.p, nblk_box, code[0].p, idx[0].p, root[b].p);
   CC_HIP(hipGetLastError());
   const uint32_t nblk = grid_for(m, SORT_TILE);
   int src = 0;
@@ -443,34 +422,39 @@ void Body::build(hipStream_t s) {
     CC_HIP(hipGetLastError());
     src ^= 1;
   }
-  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(code[src].p, idx[src].p, tris.p, m, keys.p, leaf_tri.p);
+  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(code[src].p, idx[src].p, tris.p, m, keys.p, leaf_tri[b].p);
   CC_HIP(hipGetLastError());
   if (m >= 2) {
-    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(keys.p, (int32_t)m, nodes.p, leaf_parent.p);
+    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(keys.p, (int32_t)m, nodes[b].p, leaf_parent.p, top_count.p);
     CC_HIP(hipGetLastError());
   }
-  refit(s);
+  top_setup(b, s);
+  refit(b, s);
 }
 
-void Body::refit(hipStream_t s) {
-  if (mode == CC_BODY_NONE || m == 0) return;
-  k_refit<<<grid_for(m, 256), 256, 0, s>>>(verts.p, leaf_tri.p, leaf_parent.p, nodes.p, counters.p, m, depth,
-                                             root.p, leaf0_box.p);
-  CC_HIP(hipGetLastError());
+void Body::mirror_topology(int from, hipStream_t s) {
+  if (!moving() || m == 0) return;
+  const int to = from ^ 1;
+  CC_HIP(hipMemcpyAsync(leaf_tri[to].p, leaf_tri[from].p, m * sizeof(uint4), hipMemcpyDeviceToDevice, s));
+  CC_HIP(hipMemcpyAsync(nodes[to].p, nodes[from].p, nodes[from].n * sizeof(Node), hipMemcpyDeviceToDevice, s));
+  CC_HIP(hipMemcpyAsync(root[to].p, root[from].p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  CC_HIP(hipMemcpyAsync(leaf0_box[to].p, leaf0_box[from].p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  CC_HIP(hipMemcpyAsync(verts[to].p, verts[from].p, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
-void Body::frame_update(const uint64_t* frame_dev, double dt, hipStream_t s) {
+void Body::update(int b, hipStream_t s) {
   if (mode == CC_BODY_REFIT) {
-    deform(frame_dev, dt, s);
-    refit(s);
+    deform(b, s);
+    refit(b, s);
   } else if (mode == CC_BODY_REBUILD) {
-    deform(frame_dev, dt, s);
-    build(s);
+    deform(b, s);
+    build(b, s);
   }
 }
 
-void Body::export_tree(uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB, int32_t* parent_int,
-                       int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6, hipStream_t s) {
+void Body::export_tree(int b, uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB,
+                       int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6,
+                       hipStream_t s) {
   CC_REQUIRE(mode != CC_BODY_NONE, "bvh export: no body");
   std::vector<uint64_t> k(m);
   std::vector<uint4> lt(m);
@@ -478,11 +462,12 @@ void Body::export_tree(uint64_t* okeys, uint32_t* order, int32_t* childA, int32_
   std::vector<Node> nd(m >= 2 ? m - 1 : 0);
   float r[6], l0[6];
   CC_HIP(hipMemcpyAsync(k.data(), keys.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri.p, m * sizeof(uint4), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri[b].p, m * sizeof(uint4), hipMemcpyDeviceToHost, s));
   CC_HIP(hipMemcpyAsync(lp.data(), leaf_parent.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  if (!nd.empty()) CC_HIP(hipMemcpyAsync(nd.data(), nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(r, root.p, sizeof(r), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(l0, leaf0_box.p, sizeof(l0), hipMemcpyDeviceToHost, s));
+  if (!nd.empty())
+    CC_HIP(hipMemcpyAsync(nd.data(), nodes[b].p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(r, root[b].p, sizeof(r), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(l0, leaf0_box[b].p, sizeof(l0), hipMemcpyDeviceToHost, s));
   CC_HIP(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < m; ++i) {
     if (okeys) okeys[i] = k[i];

@@ -1,0 +1,469 @@
+// refit.hip -- bottom-up LBVH box refit on gfx950.
+//
+// calculateBoudingBox (cudaBRT.cu:166-206) gives every internal node the box
+// sentinel U A U B, computed bottom-up by the second of its two children to
+// arrive.  Here the same boxes are computed level by level:
+//
+//  * The sorted leaves are cut into chunks of RC_CHUNK.  An internal node
+//    whose leaf range lies inside one chunk is "in-chunk"; the others (their
+//    range crosses a chunk boundary) are "top" nodes.
+//  * Once per topology (after a build) the setup kernels record, for every
+//    in-chunk node, its height inside the chunk (1 + the larger height of
+//    its internal children), for every top node its height inside the top
+//    tree, the list of top nodes with their links, and which nodes and
+//    leaves hang directly below a top node.
+//  * Per refit, k_refit_chunks has one workgroup per chunk compute leaf boxes
+//    into LDS and then in-chunk boxes one height level per barrier; the
+//    children of a level-h node all have smaller heights.  k_refit_top does
+//    the same for the top nodes in one workgroup.  No atomics and no
+//    cross-workgroup synchronisation: on MI355X an agent-scope arrival
+//    protocol pays an L2 write-back / invalidate per level (each XCD has its
+//    own L2); the kernel boundary between the passes gives the visibility.
+//  * All boxes go to their parent's child slot (the layout the traversal
+//    reads).  fmin/fmax unions are exact, so the order does not matter.
+#include "cc_body.h"
+
+namespace cc {
+
+constexpr int RC_CHUNK = 2048;
+constexpr int RC_THREADS = 512;
+constexpr int RC_PER_THREAD = RC_CHUNK / RC_THREADS;
+constexpr int TOP_THREADS = 1024;
+constexpr int TOP_REFIT_THREADS = 256;
+constexpr int TOP_MAX_LDS = 4096;                                   // entries held in LDS (36 B each)
+constexpr int TOP_MAX_LEVELS = 64;                                  // top-tree height bound (depth < 64)
+constexpr uint8_t BELOW_TOP = 0x80;                                 // flag: parent is a top node
+constexpr uint8_t IS_A = 0x40;                                      // flag: child A of the parent
+constexpr uint8_t H_MASK = 0x3f;                                    // in-chunk height (depth < 64)
+
+__device__ inline void put6(float* d, const float* mn, const float* mx) {
+  d[0] = mn[0]; d[1] = mn[1]; d[2] = mn[2];
+  d[3] = mx[0]; d[4] = mx[1]; d[5] = mx[2];
+}
+
+__device__ inline void unite(float* mn, float* mx, const float* b6) {
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = fminf(mn[c], b6[c]);
+    mx[c] = fmaxf(mx[c], b6[3 + c]);
+  }
+}
+
+// ======================================================== topology setup
+// Heights are found with an arrival climb (run once per topology, so its
+// latency does not matter): the second child to arrive at a node knows both
+// child heights.
+
+// top_pos[x] = list position of top node x, -1 for every other internal node
+__global__ __launch_bounds__(RC_THREADS) void k_top_list(const Node* __restrict__ nodes, uint32_t m,
+                                                         int32_t* __restrict__ top_list,
+                                                         int32_t* __restrict__ top_pos,
+                                                         uint32_t* __restrict__ top_count) {
+  __shared__ uint32_t s_ntop, s_base;
+  const int32_t c0 = (int32_t)(blockIdx.x * RC_CHUNK);
+  const int32_t i1 = min((int32_t)m - 1, c0 + RC_CHUNK);
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_ntop = 0;
+  __syncthreads();
+  bool top[RC_PER_THREAD];
+  uint32_t loc[RC_PER_THREAD];
+  for (int q = 0; q < RC_PER_THREAD; ++q) {
+    const int32_t x = c0 + q * RC_THREADS + (int32_t)tid;
+    top[q] = false;
+    if (x >= i1) continue;
+    const int32_t o = nodes[x].other;
+    top[q] = o < c0 || o >= c0 + RC_CHUNK;
+    if (top[q]) loc[q] = atomicAdd(&s_ntop, 1u);
+  }
+  __syncthreads();
+  if (tid == 0) s_base = s_ntop ? atomicAdd(top_count, s_ntop) : 0u;
+  __syncthreads();
+  for (int q = 0; q < RC_PER_THREAD; ++q) {
+    const int32_t x = c0 + q * RC_THREADS + (int32_t)tid;
+    if (x >= i1) continue;
+    int32_t pos = -1;
+    if (top[q]) {
+      pos = (int32_t)(s_base + loc[q]);
+      top_list[pos] = x;
+    }
+    top_pos[x] = pos;
+  }
+}
+
+// in-chunk heights (low 7 bits of nflag) and the below-top flags of nodes
+// and leaves; chunk_h[w] = largest in-chunk height of chunk w
+__global__ __launch_bounds__(RC_THREADS) void k_chunk_heights(const Node* __restrict__ nodes,
+                                                              const int32_t* __restrict__ leaf_parent, uint32_t m,
+                                                              const int32_t* __restrict__ top_pos,
+                                                              uint8_t* __restrict__ nflag,
+                                                              uint8_t* __restrict__ lflag,
+                                                              uint32_t* __restrict__ chunk_h) {
+  __shared__ int32_t spar[RC_CHUNK], soth[RC_CHUNK];
+  __shared__ uint32_t scnt[RC_CHUNK], shA[RC_CHUNK], shB[RC_CHUNK];
+  __shared__ int32_t sca[RC_CHUNK];
+  __shared__ uint32_t s_hmax;
+  const int32_t c0 = (int32_t)(blockIdx.x * RC_CHUNK);
+  const int32_t c1 = min((int32_t)m, c0 + RC_CHUNK);
+  const int32_t i1 = min((int32_t)m - 1, c0 + RC_CHUNK);
+  const uint32_t tid = threadIdx.x;
+  auto inside = [&](int32_t v) { return v >= c0 && v < c0 + RC_CHUNK; };
+  if (tid == 0) s_hmax = 0;
+  for (int32_t x = c0 + (int32_t)tid; x < i1; x += RC_THREADS) {
+    const int4 L = reinterpret_cast<const int4*>(nodes + x)[3];
+    sca[x - c0] = L.x;
+    spar[x - c0] = L.z;
+    soth[x - c0] = L.w;
+    scnt[x - c0] = 0;
+  }
+  __syncthreads();
+  auto in_chunk = [&](int32_t x) { return inside(x) && inside(soth[x - c0]); };
+  // one climb per leaf, height 0 at the leaf
+  for (int32_t l = c0 + (int32_t)tid; l < c1; l += RC_THREADS) {
+    int32_t x = leaf_parent[l], child = ~l;
+    uint32_t h = 0;
+    lflag[l] = in_chunk(x) ? 0 : (uint8_t)(BELOW_TOP | (nodes[x].childA == child ? IS_A : 0));
+    while (in_chunk(x)) {
+      const bool isA = sca[x - c0] == child;
+      (isA ? shA : shB)[x - c0] = h;
+      __threadfence_block();
+      if (atomicAdd(&scnt[x - c0], 1u) == 0) break;
+      __threadfence_block();
+      h = 1 + max(shA[x - c0], shB[x - c0]);
+      const int32_t p = spar[x - c0];
+      uint8_t f = (uint8_t)h;
+      if (p < 0 || !in_chunk(p)) f |= BELOW_TOP | ((p >= 0 && nodes[p].childA == x) ? IS_A : 0);
+      nflag[x] = f;
+      atomicMax(&s_hmax, h);
+      child = x;
+      x = p;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) chunk_h[blockIdx.x] = s_hmax;
+  // top nodes carry no in-chunk height
+  for (int32_t x = c0 + (int32_t)tid; x < i1; x += RC_THREADS)
+    if (!in_chunk(x)) nflag[x] = 0;
+  (void)top_pos;
+}
+
+// links[i] = {position of top child A or -1, of top child B or -1, of the
+// parent or -1, parent slot 2*parent + (A ? 0 : 1) or -1 at the root}
+__global__ __launch_bounds__(256) void k_top_links(const Node* __restrict__ nodes,
+                                                   const int32_t* __restrict__ top_list,
+                                                   const int32_t* __restrict__ top_pos,
+                                                   const uint32_t* __restrict__ top_count,
+                                                   int4* __restrict__ links) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= *top_count) return;
+  const int32_t x = top_list[i];
+  const Node& n = nodes[x];
+  const int32_t pa = n.childA >= 0 ? top_pos[n.childA] : -1;
+  const int32_t pb = n.childB >= 0 ? top_pos[n.childB] : -1;
+  int32_t pp = -1, slot = -1;
+  if (n.parent >= 0) {
+    pp = top_pos[n.parent];
+    slot = 2 * n.parent + (nodes[n.parent].childA == x ? 0 : 1);
+  }
+  links[i] = make_int4(pa, pb, pp, slot);
+}
+
+// top-tree heights (1 + larger height of top children, top leaves of the top
+// tree have height 1); one workgroup, arrival climb on counters in `work`
+__global__ __launch_bounds__(TOP_THREADS) void k_top_heights(const int4* __restrict__ links,
+                                                             const uint32_t* __restrict__ top_count,
+                                                             uint32_t* __restrict__ work,
+                                                             uint8_t* __restrict__ top_h,
+                                                             uint32_t* __restrict__ top_hmax) {
+  __shared__ uint32_t s_hmax;
+  const uint32_t T = *top_count;
+  uint32_t* cnt = work;
+  uint32_t* hA = work + T;
+  uint32_t* hB = work + 2 * (size_t)T;
+  if (threadIdx.x == 0) s_hmax = 0;
+  for (uint32_t i = threadIdx.x; i < T; i += TOP_THREADS) {
+    const int4 L = links[i];
+    cnt[i] = (L.x < 0 ? 1u : 0u) + (L.y < 0 ? 1u : 0u);
+    hA[i] = 0;
+    hB[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i0 = threadIdx.x; i0 < T; i0 += TOP_THREADS) {
+    int4 L = links[i0];
+    if (L.x >= 0 || L.y >= 0) continue;
+    uint32_t i = i0, h = 1;
+    while (true) {
+      top_h[i] = (uint8_t)h;
+      atomicMax(&s_hmax, h);
+      const int32_t j = L.z;
+      if (j < 0) break;
+      const int4 P = links[j];
+      (P.x == (int32_t)i ? hA : hB)[j] = h;
+      __threadfence_block();
+      if (atomicAdd(&cnt[j], 1u) != 1) break;
+      __threadfence_block();
+      h = 1 + max(hA[j], hB[j]);
+      i = (uint32_t)j;
+      L = P;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *top_hmax = s_hmax;
+}
+
+// ======================================================== per-frame refit
+__global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
+    const float* __restrict__ verts, const uint4* __restrict__ leaf_tri, const int32_t* __restrict__ leaf_parent,
+    Node* __restrict__ nodes, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
+    const uint8_t* __restrict__ lflag, const uint32_t* __restrict__ chunk_h, const int32_t* __restrict__ top_pos,
+    float* __restrict__ top_child, float* __restrict__ root_out, float* __restrict__ leaf0_box) {
+  __shared__ float lbox[RC_CHUNK][6];
+  __shared__ float nbox[RC_CHUNK][6];
+  const int32_t c0 = (int32_t)(blockIdx.x * RC_CHUNK);
+  const int32_t c1 = min((int32_t)m, c0 + RC_CHUNK);            // leaves [c0, c1)
+  const int32_t i1 = min((int32_t)m - 1, c0 + RC_CHUNK);        // internal nodes [c0, i1)
+  const uint32_t tid = threadIdx.x;
+  const uint32_t H = m > 1 ? chunk_h[blockIdx.x] : 0;
+
+  // leaf boxes: the loads of all of the thread's leaves first
+  uint4 tri[RC_PER_THREAD];
+  for (int r = 0; r < RC_PER_THREAD; ++r) {
+    const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
+    tri[r] = l < c1 ? leaf_tri[l] : make_uint4(0, 0, 0, 0);
+  }
+  // topology of the thread's internal nodes
+  int4 L[RC_PER_THREAD];
+  uint8_t fl[RC_PER_THREAD];
+  for (int r = 0; r < RC_PER_THREAD; ++r) {
+    const int32_t x = c0 + r * RC_THREADS + (int32_t)tid;
+    L[r] = make_int4(0, 0, -1, 0);
+    fl[r] = 0;
+    if (x < i1) {
+      L[r] = reinterpret_cast<const int4*>(nodes + x)[3];
+      fl[r] = nflag[x];
+    }
+  }
+  float vtx[RC_PER_THREAD][9];
+  for (int r = 0; r < RC_PER_THREAD; ++r) {
+    const uint32_t t3[3] = {tri[r].x, tri[r].y, tri[r].z};
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 3; ++c) vtx[r][3 * k + c] = verts[3 * (size_t)t3[k] + c];
+  }
+  for (int r = 0; r < RC_PER_THREAD; ++r) {
+    const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
+    if (l >= c1) continue;
+    float mn[3], mx[3];
+    prim_box(ld3(vtx[r]), ld3(vtx[r] + 3), ld3(vtx[r] + 6), depth, mn, mx);
+    put6(lbox[l - c0], mn, mx);
+  }
+  if (m == 1) {
+    if (tid == 0) {
+      for (int c = 0; c < 6; ++c) leaf0_box[c] = lbox[0][c];
+      float rmn[3] = {100.0f, 100.0f, 100.0f}, rmx[3] = {-100.0f, -100.0f, -100.0f};
+      unite(rmn, rmx, lbox[0]);
+      put6(root_out, rmn, rmx);
+    }
+    return;
+  }
+  __syncthreads();
+  // in-chunk boxes, one height level per barrier
+  for (uint32_t h = 1; h <= H; ++h) {
+    for (int r = 0; r < RC_PER_THREAD; ++r) {
+      if ((fl[r] & H_MASK) != h) continue;
+      const int32_t x = c0 + r * RC_THREADS + (int32_t)tid;
+      float mn[3] = {100.0f, 100.0f, 100.0f}, mx[3] = {-100.0f, -100.0f, -100.0f};
+      unite(mn, mx, L[r].x < 0 ? lbox[~L[r].x - c0] : nbox[L[r].x - c0]);
+      unite(mn, mx, L[r].y < 0 ? lbox[~L[r].y - c0] : nbox[L[r].y - c0]);
+      put6(nbox[x - c0], mn, mx);
+    }
+    __syncthreads();
+  }
+  // stores: both child slots of every in-chunk node; boxes of nodes / leaves
+  // directly below a top node to that node's slot and its top-pass record
+  for (int r = 0; r < RC_PER_THREAD; ++r) {
+    const int32_t x = c0 + r * RC_THREADS + (int32_t)tid;
+    if (x < i1 && (fl[r] & H_MASK)) {
+      const float* a = L[r].x < 0 ? lbox[~L[r].x - c0] : nbox[L[r].x - c0];
+      const float* b = L[r].y < 0 ? lbox[~L[r].y - c0] : nbox[L[r].y - c0];
+      float4* d = reinterpret_cast<float4*>(nodes + x);
+      d[0] = make_float4(a[0], a[1], a[2], a[3]);
+      d[1] = make_float4(a[4], a[5], b[0], b[1]);
+      d[2] = make_float4(b[2], b[3], b[4], b[5]);
+      if (fl[r] & BELOW_TOP) {
+        const int32_t p = L[r].z;
+        const float* bx = nbox[x - c0];
+        if (p < 0) {                      // the whole tree lies in this chunk
+          for (int c = 0; c < 6; ++c) root_out[c] = bx[c];
+        } else {
+          const bool isA = (fl[r] & IS_A) != 0;
+          put6(isA ? nodes[p].a_min : nodes[p].b_min, bx, bx + 3);
+          put6(top_child + 12 * (size_t)top_pos[p] + (isA ? 0 : 6), bx, bx + 3);
+        }
+      }
+    }
+    const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
+    if (l < c1) {
+      const uint8_t lf = lflag[l];
+      if (lf & BELOW_TOP) {
+        const int32_t p = leaf_parent[l];
+        const bool isA = (lf & IS_A) != 0;
+        put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
+        put6(top_child + 12 * (size_t)top_pos[p] + (isA ? 0 : 6), lbox[l - c0], lbox[l - c0] + 3);
+      }
+    }
+  }
+}
+
+// Setup, one workgroup: renumber the top entries by height (counting sort)
+// so that each level is a contiguous range [lev[h], lev[h+1]); positions in
+// top_list / top_pos / top_links are rewritten to the sorted numbering.
+__global__ __launch_bounds__(TOP_THREADS) void k_top_sort(int32_t* __restrict__ top_list,
+                                                          int32_t* __restrict__ top_pos,
+                                                          int4* __restrict__ links,
+                                                          const uint8_t* __restrict__ top_h,
+                                                          const uint32_t* __restrict__ top_count,
+                                                          const uint32_t* __restrict__ top_hmax,
+                                                          uint32_t* __restrict__ lev, int32_t* __restrict__ work) {
+  __shared__ uint32_t cnt[TOP_MAX_LEVELS + 2], fill[TOP_MAX_LEVELS + 2];
+  const uint32_t T = *top_count, H = *top_hmax, tid = threadIdx.x;
+  int32_t* old_list = work;                                       // [T]
+  int4* old_links = reinterpret_cast<int4*>(work + T + (T & 3 ? 4 - (T & 3) : 0));   // [T], 16 B aligned
+  int32_t* newpos = reinterpret_cast<int32_t*>(old_links + T);   // [T]
+  for (uint32_t h = tid; h < TOP_MAX_LEVELS + 2; h += TOP_THREADS) {
+    cnt[h] = 0;
+    fill[h] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < T; i += TOP_THREADS) {
+    old_list[i] = top_list[i];
+    old_links[i] = links[i];
+    atomicAdd(&cnt[top_h[i]], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (uint32_t h = 1; h <= H + 1; ++h) {
+      const uint32_t c = h <= H ? cnt[h] : 0;
+      cnt[h] = run;                   // start of level h
+      lev[h] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < T; i += TOP_THREADS) {
+    const uint32_t h = top_h[i];
+    newpos[i] = (int32_t)(cnt[h] + atomicAdd(&fill[h], 1u));
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < T; i += TOP_THREADS) {
+    const int32_t p = newpos[i];
+    const int4 L = old_links[i];
+    top_list[p] = old_list[i];
+    top_pos[old_list[i]] = p;
+    links[p] = make_int4(L.x >= 0 ? newpos[L.x] : -1, L.y >= 0 ? newpos[L.y] : -1,
+                         L.z >= 0 ? newpos[L.z] : -1, L.w);
+  }
+}
+
+// Per refit, one workgroup: entry e's box = sentinel U its non-top children
+// (top_child) U its top children (lower levels); one level per barrier, one
+// entry per thread within a level.
+template <bool LDS>
+__device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t H, const uint32_t* __restrict__ lev,
+                                 const int4* __restrict__ links, const float* __restrict__ top_child,
+                                 float* __restrict__ root_out, float* box, int2* lnk, int32_t* lsl) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < T; i += TOP_REFIT_THREADS) {
+    const int4 L = links[i];
+    const float* tc = top_child + 12 * (size_t)i;
+    float mn[3] = {100.0f, 100.0f, 100.0f}, mx[3] = {-100.0f, -100.0f, -100.0f};
+    if (L.x < 0) unite(mn, mx, tc);
+    if (L.y < 0) unite(mn, mx, tc + 6);
+    put6(box + 6 * (size_t)i, mn, mx);
+    lnk[i] = make_int2(L.x, L.y);
+    lsl[i] = L.w;
+  }
+  __syncthreads();
+  for (uint32_t h = 1; h <= H; ++h) {
+    const uint32_t e1 = lev[h + 1];
+    for (uint32_t e = lev[h] + tid; e < e1; e += TOP_REFIT_THREADS) {
+      const int2 c = lnk[e];
+      float* be = box + 6 * (size_t)e;
+      float mn[3] = {be[0], be[1], be[2]}, mx[3] = {be[3], be[4], be[5]};
+      if (c.x >= 0) unite(mn, mx, box + 6 * (size_t)c.x);
+      if (c.y >= 0) unite(mn, mx, box + 6 * (size_t)c.y);
+      put6(be, mn, mx);
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < T; i += TOP_REFIT_THREADS) {
+    const int32_t s = lsl[i];
+    float* d = s < 0 ? root_out : (s & 1 ? nodes[s >> 1].b_min : nodes[s >> 1].a_min);
+    const float* b = box + 6 * (size_t)i;
+    for (int c = 0; c < 6; ++c) d[c] = b[c];
+  }
+}
+
+__global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restrict__ nodes,
+                                                                 const int4* __restrict__ links,
+                                                                 const uint32_t* __restrict__ lev,
+                                                                 const float* __restrict__ top_child,
+                                                                 const uint32_t* __restrict__ top_count,
+                                                                 const uint32_t* __restrict__ top_hmax,
+                                                                 float* root_out, float* __restrict__ scratch) {
+  __shared__ float sbox[TOP_MAX_LDS * 6];
+  __shared__ int2 slnk[TOP_MAX_LDS];
+  __shared__ int32_t ssl[TOP_MAX_LDS];
+  __shared__ uint32_t slev[TOP_MAX_LEVELS + 2];
+  const uint32_t T = *top_count, H = *top_hmax;
+  if (T == 0) return;
+  for (uint32_t h = threadIdx.x; h <= H + 1; h += TOP_REFIT_THREADS) slev[h] = lev[h];
+  __syncthreads();
+  if (T <= (uint32_t)TOP_MAX_LDS) {
+    refit_top_levels<true>(nodes, T, H, slev, links, top_child, root_out, sbox, slnk, ssl);
+  } else {
+    // more entries than LDS holds: the same levels on global scratch (one
+    // workgroup, so the barriers order it)
+    float* b = scratch;
+    int2* l = reinterpret_cast<int2*>(b + 6 * (size_t)T);
+    int32_t* sl = reinterpret_cast<int32_t*>(l + T);
+    refit_top_levels<false>(nodes, T, H, slev, links, top_child, root_out, b, l, sl);
+  }
+}
+
+// ======================================================== host side
+void Body::top_setup(int b, hipStream_t s) {
+  // top_count was zeroed by k_karras (no memset node in captured graphs)
+  if (mode == CC_BODY_NONE || m < 2) return;
+  const unsigned nchunk = grid_for(m, RC_CHUNK);
+  Node* nd = nodes[b].p;
+  k_top_list<<<nchunk, RC_THREADS, 0, s>>>(nd, m, top_list.p, top_pos.p, top_count.p);
+  CC_HIP(hipGetLastError());
+  k_chunk_heights<<<nchunk, RC_THREADS, 0, s>>>(nd, leaf_parent.p, m, top_pos.p, nflag.p, lflag.p, chunk_h.p);
+  CC_HIP(hipGetLastError());
+  if (m > (uint32_t)RC_CHUNK) {
+    k_top_links<<<grid_for(m, 256), 256, 0, s>>>(nd, top_list.p, top_pos.p, top_count.p, top_links.p);
+    CC_HIP(hipGetLastError());
+    k_top_heights<<<1, TOP_THREADS, 0, s>>>(top_links.p, top_count.p, reinterpret_cast<uint32_t*>(top_scratch.p),
+                                            top_h.p, top_hmax.p);
+    CC_HIP(hipGetLastError());
+    k_top_sort<<<1, TOP_THREADS, 0, s>>>(top_list.p, top_pos.p, top_links.p, top_h.p, top_count.p, top_hmax.p,
+                                         top_lev.p, reinterpret_cast<int32_t*>(top_scratch.p));
+    CC_HIP(hipGetLastError());
+  }
+}
+
+void Body::refit(int b, hipStream_t s) {
+  if (mode == CC_BODY_NONE || m == 0) return;
+  k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(verts[b].p, leaf_tri[b].p, leaf_parent.p, nodes[b].p,
+                                                              m, depth, nflag.p, lflag.p, chunk_h.p, top_pos.p,
+                                                              top_child.p, root[b].p, leaf0_box[b].p);
+  CC_HIP(hipGetLastError());
+  if (m > (uint32_t)RC_CHUNK) {
+    k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(nodes[b].p, top_links.p, top_lev.p, top_child.p, top_count.p,
+                                                top_hmax.p, root[b].p, top_scratch.p);
+    CC_HIP(hipGetLastError());
+  }
+}
+
+size_t Body::top_scratch_floats(uint32_t m) { return 9 * (size_t)(m > 1 ? m - 1 : 1) + 64; }
+
+}  // namespace cc

