@@ -100,16 +100,22 @@ def main():
 
     sc = S.large()
     sim = Simulator.from_scene(sc)
+    # HIP events around every step graph and update-lane graph, on the
+    # streams they run on, read while the following blocks run
+    sim.set_timing(True)
     sim.simulate(args.warmup)
     sim.synchronize()
+    sim.set_timing(True)                  # averages over the timed steps only
     t0 = time.perf_counter()
     sim.simulate(args.steps)
     sim.synchronize()
     el = time.perf_counter() - t0
     ms = 1e3 * el / args.steps
     value = sc.n * args.steps / el
+    live = sim.timing()
+    sim.set_timing(False)
 
-    # per-kernel durations with HIP events on the simulator stream
+    # the same kernel groups one at a time, nothing overlapped
     tk = sim.time_kernels(args.steps)
     # traversal counters over a separate run of the same steps
     sim.set_counting(True)
@@ -122,8 +128,10 @@ def main():
     sim.check_finite()
 
     ab = algorithmic_bytes(sc)
-    kern = {"verlet": tk["verlet_us"], "refit": tk["refit_us"], "deform": tk["body_us"]}
-    dom = max(kern, key=kern.get)
+    kern = {"verlet": live["verlet_us"], "body_update": live["refit_us"]}
+    iso = {"verlet": tk["verlet_us"], "refit": tk["refit_us"], "deform": tk["body_us"]}
+    # the step chain is the critical path: the body updates run beside it
+    dom = "verlet"
     achieved = ab[dom] / (kern[dom] * 1e-6) / 1e9
     step_bytes = ab["verlet"] + ab["refit"] + ab["deform"]
     out = {
@@ -147,7 +155,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels_us": kern,
-        "step_us_graph": tk["step_us"],
+        "kernels_timed_steps": live["steps"],
+        "kernels_isolated_us": iso,
         "step_algorithmic_bytes": step_bytes,
         "step_hbm_frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "bvh_nodes_visited_per_s": visits / el_c,

@@ -264,16 +264,31 @@ int cc_sim_get_normals(cc_sim* sim, float* normals3);
 /* ----------------------------------------------------------------- timing */
 
 /* Per-kernel timing of nsteps steps with HIP events on the simulator
-   stream (each kernel bracketed), averaged per launch in microseconds. */
+   stream, one kernel group at a time (each bracketed, nothing overlapped),
+   averaged per launch in microseconds; step_us is then the per-step time of
+   nsteps further steps through the normal (overlapped) path. */
 typedef struct cc_timing {
   double step_us;      /* whole step (graph replay), per step           */
   double verlet_us;    /* fused Verlet + spring + collision kernel      */
   double body_us;      /* deformation kernel                            */
-  double refit_us;     /* refit kernel                                  */
+  double refit_us;     /* refit kernels                                 */
   double build_us;     /* rebuild kernels (boxes, sort, Karras) summed  */
   uint32_t steps;
 } cc_timing;
 int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
+
+/* Timing inside the normal path: while enabled, cc_sim_step records HIP
+   events around each graph of fused step kernels and around each body update
+   lane's launches, on the streams they run on, and accumulates their
+   durations (two event sets alternate, so the host reads one block's events
+   while the next block runs).
+   cc_sim_get_timing returns the averages since enabling: verlet_us per
+   fused step kernel, refit_us (refitted body) or build_us (rebuilt body)
+   per body update including its deformation, steps = steps timed.
+   Calling it again resets the averages.  No counterpart in the reference
+   (measurement only). */
+int cc_sim_set_timing(cc_sim* sim, int enable);
+int cc_sim_get_timing(cc_sim* sim, cc_timing* out);
 
 #ifdef __cplusplus
 }

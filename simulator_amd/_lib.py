@@ -121,6 +121,8 @@ PROTOTYPES = {
     "cc_sim_write_vbo": (C.c_int, [P, P, C.c_size_t, u32, C.c_int]),
     "cc_sim_get_normals": (C.c_int, [P, P]),
     "cc_sim_time_kernels": (C.c_int, [P, C.c_int, C.POINTER(cc_timing)]),
+    "cc_sim_set_timing": (C.c_int, [P, C.c_int]),
+    "cc_sim_get_timing": (C.c_int, [P, C.POINTER(cc_timing)]),
 }
 
 _lib = None

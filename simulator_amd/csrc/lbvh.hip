@@ -16,34 +16,30 @@
 namespace cc {
 
 // ------------------------------------------------------------ deformation
-// k_motion computes the frame's parameters once (one block) from the body's
-// frame counter and advances the counter; then every k_deform thread moves
-// one vertex: radial breathing about the closest point of its capsule axis,
-// then R * Y + T.
-constexpr int MAX_CAPS = 64;
-
-__global__ __launch_bounds__(64) void k_motion(MotionArgs a) {
-  const uint64_t f = *a.frame;
-  __syncthreads();
-  compute_motion(a, f, threadIdx.x, blockDim.x);
-  if (threadIdx.x == 0) *a.frame = f + 1;
-}
-
+// Every block computes the frame's motion parameters into LDS (block 0 also
+// stores them for the host), then every thread moves one vertex: radial
+// breathing about the closest point of its capsule axis, then R * Y + T.
+// The frame is (*base if base else 0) + off.
 __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, float* __restrict__ out,
-                                                uint32_t nv, uint32_t ncaps,
-                                                const uint32_t* __restrict__ vstart,
+                                                uint32_t nv, const uint32_t* __restrict__ vstart,
                                                 const float* __restrict__ p0, const float* __restrict__ p1,
-                                                const float* __restrict__ par) {
+                                                const float* __restrict__ phase, MotionParams mp,
+                                                const uint64_t* __restrict__ base, uint64_t off,
+                                                float* __restrict__ motion_out) {
   __shared__ float sP[12 + MAX_CAPS], sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
   __shared__ uint32_t sV[MAX_CAPS + 1];
   const uint32_t tid = threadIdx.x;
-  for (uint32_t k = tid; k < 12 + ncaps; k += blockDim.x) sP[k] = par[k];
+  const uint32_t ncaps = mp.ncaps;
+  const uint64_t frame = (base ? *base : 0ull) + off;
+  compute_motion(mp, phase, frame, sP, tid, blockDim.x);
   for (uint32_t k = tid; k < 3 * ncaps; k += blockDim.x) {
     sP0[k] = p0[k];
     sP1[k] = p1[k];
   }
   for (uint32_t k = tid; k <= ncaps; k += blockDim.x) sV[k] = vstart[k];
   __syncthreads();
+  if (blockIdx.x == 0)
+    for (uint32_t k = tid; k < 12 + ncaps; k += blockDim.x) motion_out[k] = sP[k];
   const uint32_t v = blockIdx.x * blockDim.x + tid;
   if (v >= nv) return;
   // capsule of v: last k with vstart[k] <= v
@@ -300,37 +296,12 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ key
 }
 
 // ------------------------------------------------------------ host side
-void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, uint32_t mode_, float depth_,
-                const cc_capsules* caps, const MotionDesc& md, hipStream_t s) {
-  CC_REQUIRE(mode_ <= CC_BODY_REBUILD, "body: unknown mode");
-  mode = mode_;
-  nv = nv_;
-  m = nf;
-  depth = depth_;
-  motion = md;
-  if (mode == CC_BODY_NONE) return;
-  CC_REQUIRE(v3 && t3 && nv > 0, "body: vertices and triangles required");
-  CC_REQUIRE(m < (1u << 30), "body: too many triangles");
-  for (size_t i = 0; i < 3 * (size_t)m; ++i) CC_REQUIRE(t3[i] < nv, "body: triangle index out of range");
-  const int nbuf = moving() ? 2 : 1;
-  rest.upload(v3, 3 * (size_t)nv, s);
-  for (int b = 0; b < nbuf; ++b) verts[b].upload(v3, 3 * (size_t)nv, s);
-  tris.upload(t3, 3 * (size_t)m, s);
-  frame_ctr.alloc(1);
-  frame_ctr.zero(s);
-  if (moving()) {
-    CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
-    CC_REQUIRE(caps->vstart[0] == 0 && caps->vstart[caps->ncaps] == nv, "body: capsule ranges must cover vertices");
-    for (uint32_t k = 0; k < caps->ncaps; ++k)
-      CC_REQUIRE(caps->vstart[k] <= caps->vstart[k + 1], "body: capsule ranges must be ascending");
-    ncaps = caps->ncaps;
-    cap_vstart.upload(caps->vstart, ncaps + 1, s);
-    cap_p0.upload(caps->p0, 3 * (size_t)ncaps, s);
-    cap_p1.upload(caps->p1, 3 * (size_t)ncaps, s);
-    cap_phase.upload(caps->phase, ncaps, s);
-    motion_cur.alloc(12 + ncaps);
-    motion_cur.zero(s);
-  }
+void Lane::alloc(uint32_t m, bool full_lane, hipStream_t s) {
+  full = full_lane;
+  const size_t mi = std::max<uint32_t>(m, 2) - 1;
+  top_child.alloc(12 * mi);
+  top_scratch.alloc(Body::top_scratch_floats(m));
+  if (!full) return;
   nblk_box = grid_for(m, BOX_THREADS);
   boxes.alloc(6 * (size_t)m);
   partial.alloc(6 * (size_t)nblk_box);
@@ -341,22 +312,11 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
   keys.alloc(m);
   leaf_parent.alloc(m);
-  const size_t mi = std::max<uint32_t>(m, 2) - 1;
-  for (int b = 0; b < nbuf; ++b) {
-    leaf_tri[b].alloc(m);
-    nodes[b].alloc(mi);
-    nodes[b].zero(s);
-    root[b].alloc(6);
-    leaf0_box[b].alloc(6);
-    leaf0_box[b].zero(s);
-  }
   top_list.alloc(mi);
   top_pos.alloc(mi);
   top_count.alloc(1);
   top_count.zero(s);
   top_links.alloc(mi);
-  top_child.alloc(12 * mi);
-  top_scratch.alloc(top_scratch_floats(m));
   top_h.alloc(mi);
   top_hmax.alloc(1);
   top_hmax.zero(s);
@@ -370,107 +330,134 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   chunk_h.zero(s);
 }
 
-MotionArgs Body::motion_args() {
-  MotionArgs a;
-  a.amp = motion.amp;
-  a.freq = motion.freq;
-  a.dx = motion.dir[0];
-  a.dy = motion.dir[1];
-  a.dz = motion.dir[2];
-  a.rot_amp = motion.rot_amp;
-  a.px = motion.pivot[0];
-  a.py = motion.pivot[1];
-  a.pz = motion.pivot[2];
-  a.breath_amp = motion.breath_amp;
-  a.breath_freq = motion.breath_freq;
-  a.dt = dt;
-  a.phase = cap_phase.p;
-  a.out = motion_cur.p;
-  a.frame = frame_ctr.p;
-  a.ncaps = ncaps;
-  return a;
-}
-
-void Body::set_frame(uint64_t frame, hipStream_t s) {
+void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, uint32_t mode_, float depth_,
+                const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s) {
+  CC_REQUIRE(mode_ <= CC_BODY_REBUILD, "body: unknown mode");
+  mode = mode_;
+  nv = nv_;
+  m = nf;
+  depth = depth_;
+  motion = mp;
   if (mode == CC_BODY_NONE) return;
-  CC_HIP(hipMemcpyAsync(frame_ctr.p, &frame, sizeof(frame), hipMemcpyHostToDevice, s));
-  CC_HIP(hipStreamSynchronize(s));
+  CC_REQUIRE(v3 && t3 && nv > 0, "body: vertices and triangles required");
+  CC_REQUIRE(m < (1u << 30), "body: too many triangles");
+  for (size_t i = 0; i < 3 * (size_t)m; ++i) CC_REQUIRE(t3[i] < nv, "body: triangle index out of range");
+  nslots = moving() ? slots : 1;
+  nlanes = moving() ? lanes : 1;
+  CC_REQUIRE(nslots >= 1 && nslots <= MAX_SLOTS && nlanes >= 1 && nlanes <= MAX_LANES, "body: slot/lane count");
+  rest.upload(v3, 3 * (size_t)nv, s);
+  tris.upload(t3, 3 * (size_t)m, s);
+  if (moving()) {
+    CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
+    CC_REQUIRE(caps->vstart[0] == 0 && caps->vstart[caps->ncaps] == nv, "body: capsule ranges must cover vertices");
+    for (uint32_t k = 0; k < caps->ncaps; ++k)
+      CC_REQUIRE(caps->vstart[k] <= caps->vstart[k + 1], "body: capsule ranges must be ascending");
+    ncaps = caps->ncaps;
+    motion.ncaps = ncaps;
+    cap_vstart.upload(caps->vstart, ncaps + 1, s);
+    cap_p0.upload(caps->p0, 3 * (size_t)ncaps, s);
+    cap_p1.upload(caps->p1, 3 * (size_t)ncaps, s);
+    cap_phase.upload(caps->phase, ncaps, s);
+  }
+  const size_t mi = std::max<uint32_t>(m, 2) - 1;
+  if (mode != CC_BODY_REBUILD) leaf_tri.alloc(m);
+  for (int k = 0; k < nslots; ++k) {
+    Slot& S = slot[k];
+    if (k == 0 || moving()) S.verts.upload(v3, 3 * (size_t)nv, s);
+    if (mode == CC_BODY_REBUILD) S.leaf_tri.alloc(m);
+    S.nodes.alloc(mi);
+    S.nodes.zero(s);
+    S.root.alloc(6);
+    S.leaf0_box.alloc(6);
+    S.leaf0_box.zero(s);
+    S.motion.alloc(12 + (size_t)ncaps);
+    S.motion.zero(s);
+  }
+  for (int l = 0; l < nlanes; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, s);
 }
 
-void Body::deform(int b, hipStream_t s) {
+void Body::deform(int k, const uint64_t* base, uint64_t off, hipStream_t s) {
   if (nv == 0 || ncaps == 0) return;
-  k_motion<<<1, 64, 0, s>>>(motion_args());
-  CC_HIP(hipGetLastError());
-  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, verts[b].p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
-                                               motion_cur.p);
+  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, slot[k].verts.p, nv, cap_vstart.p, cap_p0.p, cap_p1.p,
+                                               cap_phase.p, motion, base, off, slot[k].motion.p);
   CC_HIP(hipGetLastError());
 }
 
-void Body::build(int b, hipStream_t s) {
+void Body::build(int k, int l, hipStream_t s) {
   if (mode == CC_BODY_NONE || m == 0) return;
-  k_prim_boxes<<<nblk_box, BOX_THREADS, 0, s>>>(verts[b].p, tris.p, m, depth, boxes.p, partial.p);
+  Lane& L = lane[l];
+  CC_REQUIRE(L.full, "body: build on a refit-only lane");
+  Slot& S = slot[k];
+  k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(S.verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
   CC_HIP(hipGetLastError());
-  k_morton<<<grid_for(m, 256), 256, 0, s>>>(boxes.p, m, partial.p, nblk_box, code[0].p, idx[0].p, root[b].p);
+  k_morton<<<grid_for(m, 256), 256, 0, s>>>(L.boxes.p, m, L.partial.p, L.nblk_box, L.code[0].p, L.idx[0].p,
+                                            S.root.p);
   CC_HIP(hipGetLastError());
   const uint32_t nblk = grid_for(m, SORT_TILE);
   int src = 0;
   for (int shift = 0; shift < 32; shift += 8) {
-    k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(code[src].p, m, shift, hist.p, nblk);
-    k_scan<<<1, 1024, 0, s>>>(hist.p, 256 * nblk);
-    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(code[src].p, idx[src].p, m, shift, hist.p, nblk,
-                                                  code[src ^ 1].p, idx[src ^ 1].p);
+    k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, m, shift, L.hist.p, nblk);
+    k_scan<<<1, 1024, 0, s>>>(L.hist.p, 256 * nblk);
+    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p, nblk,
+                                                  L.code[src ^ 1].p, L.idx[src ^ 1].p);
     CC_HIP(hipGetLastError());
     src ^= 1;
   }
-  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(code[src].p, idx[src].p, tris.p, m, keys.p, leaf_tri[b].p);
+  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(L.code[src].p, L.idx[src].p, tris.p, m, L.keys.p,
+                                                 leaf_tri_of(k));
   CC_HIP(hipGetLastError());
   if (m >= 2) {
-    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(keys.p, (int32_t)m, nodes[b].p, leaf_parent.p, top_count.p);
+    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(L.keys.p, (int32_t)m, S.nodes.p, L.leaf_parent.p,
+                                                  L.top_count.p);
     CC_HIP(hipGetLastError());
   }
-  top_setup(b, s);
-  refit(b, s);
+  top_setup(k, l, s);
+  refit(k, l, s);
 }
 
-void Body::mirror_topology(int from, hipStream_t s) {
-  if (!moving() || m == 0) return;
-  const int to = from ^ 1;
-  CC_HIP(hipMemcpyAsync(leaf_tri[to].p, leaf_tri[from].p, m * sizeof(uint4), hipMemcpyDeviceToDevice, s));
-  CC_HIP(hipMemcpyAsync(nodes[to].p, nodes[from].p, nodes[from].n * sizeof(Node), hipMemcpyDeviceToDevice, s));
-  CC_HIP(hipMemcpyAsync(root[to].p, root[from].p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  CC_HIP(hipMemcpyAsync(leaf0_box[to].p, leaf0_box[from].p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  CC_HIP(hipMemcpyAsync(verts[to].p, verts[from].p, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s));
-}
-
-void Body::update(int b, hipStream_t s) {
-  if (mode == CC_BODY_REFIT) {
-    deform(b, s);
-    refit(b, s);
-  } else if (mode == CC_BODY_REBUILD) {
-    deform(b, s);
-    build(b, s);
+void Body::share_topology(int from, hipStream_t s) {
+  if (mode != CC_BODY_REFIT || m == 0) return;
+  const Slot& F = slot[from];
+  for (int k = 0; k < nslots; ++k) {
+    if (k == from) continue;
+    Slot& S = slot[k];
+    CC_HIP(hipMemcpyAsync(S.nodes.p, F.nodes.p, F.nodes.n * sizeof(Node), hipMemcpyDeviceToDevice, s));
+    CC_HIP(hipMemcpyAsync(S.root.p, F.root.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    CC_HIP(hipMemcpyAsync(S.leaf0_box.p, F.leaf0_box.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
   }
 }
 
-void Body::export_tree(int b, uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB,
+void Body::update(int k, int l, const uint64_t* base, uint64_t off, hipStream_t s) {
+  if (mode == CC_BODY_REFIT) {
+    deform(k, base, off, s);
+    refit(k, l, s);
+  } else if (mode == CC_BODY_REBUILD) {
+    deform(k, base, off, s);
+    build(k, l, s);
+  }
+}
+
+void Body::export_tree(int k, uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB,
                        int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6,
                        hipStream_t s) {
   CC_REQUIRE(mode != CC_BODY_NONE, "bvh export: no body");
-  std::vector<uint64_t> k(m);
+  const Lane& L = lane[0];
+  const Slot& S = slot[k];
+  std::vector<uint64_t> kk(m);
   std::vector<uint4> lt(m);
   std::vector<int32_t> lp(m);
   std::vector<Node> nd(m >= 2 ? m - 1 : 0);
   float r[6], l0[6];
-  CC_HIP(hipMemcpyAsync(k.data(), keys.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri[b].p, m * sizeof(uint4), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(lp.data(), leaf_parent.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(kk.data(), L.keys.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri_of(k), m * sizeof(uint4), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lp.data(), L.leaf_parent.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   if (!nd.empty())
-    CC_HIP(hipMemcpyAsync(nd.data(), nodes[b].p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(r, root[b].p, sizeof(r), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(l0, leaf0_box[b].p, sizeof(l0), hipMemcpyDeviceToHost, s));
+    CC_HIP(hipMemcpyAsync(nd.data(), S.nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(r, S.root.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(l0, S.leaf0_box.p, sizeof(l0), hipMemcpyDeviceToHost, s));
   CC_HIP(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < m; ++i) {
-    if (okeys) okeys[i] = k[i];
+    if (okeys) okeys[i] = kk[i];
     if (order) order[i] = lt[i].w;
   }
   if (root6) std::memcpy(root6, r, sizeof(r));
@@ -479,7 +466,7 @@ void Body::export_tree(int b, uint64_t* okeys, uint32_t* order, int32_t* childA,
     if (parent_leaf) parent_leaf[0] = 0;
     return;
   }
-  auto slot_of = [&](int32_t par, int32_t child, float* dst) {
+  auto slot_of_child = [&](int32_t par, int32_t child, float* dst) {
     const Node& p = nd[par];
     const float* src = (p.childA == child) ? p.a_min : p.b_min;
     std::memcpy(dst, src, 6 * sizeof(float));
@@ -491,12 +478,12 @@ void Body::export_tree(int b, uint64_t* okeys, uint32_t* order, int32_t* childA,
     if (parent_int) parent_int[x] = n.parent + 1;
     if (box_int) {
       if (n.parent < 0) std::memcpy(box_int + 6 * (size_t)x, r, sizeof(r));
-      else slot_of(n.parent, (int32_t)x, box_int + 6 * (size_t)x);
+      else slot_of_child(n.parent, (int32_t)x, box_int + 6 * (size_t)x);
     }
   }
   for (uint32_t l = 0; l < m; ++l) {
     if (parent_leaf) parent_leaf[l] = lp[l] + 1;
-    if (box_leaf) slot_of(lp[l], ~(int32_t)l, box_leaf + 6 * (size_t)l);
+    if (box_leaf) slot_of_child(lp[l], ~(int32_t)l, box_leaf + 6 * (size_t)l);
   }
 }
 

@@ -430,36 +430,42 @@ __global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restric
 }
 
 // ======================================================== host side
-void Body::top_setup(int b, hipStream_t s) {
+void Body::top_setup(int k, int l, hipStream_t s) {
   // top_count was zeroed by k_karras (no memset node in captured graphs)
   if (mode == CC_BODY_NONE || m < 2) return;
+  Lane& L = lane[l];
   const unsigned nchunk = grid_for(m, RC_CHUNK);
-  Node* nd = nodes[b].p;
-  k_top_list<<<nchunk, RC_THREADS, 0, s>>>(nd, m, top_list.p, top_pos.p, top_count.p);
+  const Node* nd = slot[k].nodes.p;
+  k_top_list<<<nchunk, RC_THREADS, 0, s>>>(nd, m, L.top_list.p, L.top_pos.p, L.top_count.p);
   CC_HIP(hipGetLastError());
-  k_chunk_heights<<<nchunk, RC_THREADS, 0, s>>>(nd, leaf_parent.p, m, top_pos.p, nflag.p, lflag.p, chunk_h.p);
+  k_chunk_heights<<<nchunk, RC_THREADS, 0, s>>>(nd, L.leaf_parent.p, m, L.top_pos.p, L.nflag.p, L.lflag.p,
+                                                L.chunk_h.p);
   CC_HIP(hipGetLastError());
   if (m > (uint32_t)RC_CHUNK) {
-    k_top_links<<<grid_for(m, 256), 256, 0, s>>>(nd, top_list.p, top_pos.p, top_count.p, top_links.p);
+    k_top_links<<<grid_for(m, 256), 256, 0, s>>>(nd, L.top_list.p, L.top_pos.p, L.top_count.p, L.top_links.p);
     CC_HIP(hipGetLastError());
-    k_top_heights<<<1, TOP_THREADS, 0, s>>>(top_links.p, top_count.p, reinterpret_cast<uint32_t*>(top_scratch.p),
-                                            top_h.p, top_hmax.p);
+    k_top_heights<<<1, TOP_THREADS, 0, s>>>(L.top_links.p, L.top_count.p,
+                                            reinterpret_cast<uint32_t*>(L.top_scratch.p), L.top_h.p, L.top_hmax.p);
     CC_HIP(hipGetLastError());
-    k_top_sort<<<1, TOP_THREADS, 0, s>>>(top_list.p, top_pos.p, top_links.p, top_h.p, top_count.p, top_hmax.p,
-                                         top_lev.p, reinterpret_cast<int32_t*>(top_scratch.p));
+    k_top_sort<<<1, TOP_THREADS, 0, s>>>(L.top_list.p, L.top_pos.p, L.top_links.p, L.top_h.p, L.top_count.p,
+                                         L.top_hmax.p, L.top_lev.p, reinterpret_cast<int32_t*>(L.top_scratch.p));
     CC_HIP(hipGetLastError());
   }
 }
 
-void Body::refit(int b, hipStream_t s) {
+void Body::refit(int k, int l, hipStream_t s) {
   if (mode == CC_BODY_NONE || m == 0) return;
-  k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(verts[b].p, leaf_tri[b].p, leaf_parent.p, nodes[b].p,
-                                                              m, depth, nflag.p, lflag.p, chunk_h.p, top_pos.p,
-                                                              top_child.p, root[b].p, leaf0_box[b].p);
+  Lane& L = lane[l];
+  Lane& T = topo(l);
+  Slot& S = slot[k];
+  k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(S.verts.p, leaf_tri_of(k), T.leaf_parent.p,
+                                                              S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
+                                                              T.chunk_h.p, T.top_pos.p, L.top_child.p, S.root.p,
+                                                              S.leaf0_box.p);
   CC_HIP(hipGetLastError());
   if (m > (uint32_t)RC_CHUNK) {
-    k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(nodes[b].p, top_links.p, top_lev.p, top_child.p, top_count.p,
-                                                top_hmax.p, root[b].p, top_scratch.p);
+    k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(S.nodes.p, T.top_links.p, T.top_lev.p, L.top_child.p,
+                                                T.top_count.p, T.top_hmax.p, S.root.p, L.top_scratch.p);
     CC_HIP(hipGetLastError());
   }
 }

@@ -9,6 +9,7 @@
 //   row       4 B merged CSR row start of all spring types; bit 31 = pinned
 //   rec       8 B per directed spring {end | type << 30, rest length}
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -197,8 +198,9 @@ struct Sim {
   DBuf<uint32_t> row;
   DBuf<uint2> rec;
   std::vector<uint32_t> h_row;
-  int cur = 0;          // parity: state[cur] and body buffer buf(cur) hold frame `frame`
+  int cur = 0;              // state[cur] holds frame `frame`
   uint64_t frame = 0;
+  uint64_t ready = 0;       // moving body: the poses of frames [frame, ready) are in their slots
   bool debug = false, counting = false;
   DBuf<float> dbg_pre;
   DBuf<int32_t> dbg_hit;
@@ -209,26 +211,44 @@ struct Sim {
   uint32_t nf = 0;
   DBuf<uint32_t> faces, vrow, vcol;
   DBuf<float> face_n, vnormals;
-  // graphs per ping-pong parity: one step, and GRAPH_STEPS steps
+
+  // Graphs of GRAPH_STEPS steps, per slot of the first frame and state
+  // parity (captured when first needed).
   static constexpr int GRAPH_STEPS = 16;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraphExec_t gmulti[2] = {nullptr, nullptr};
-  // second stream and events of the pipelined body update (moving bodies)
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_body = nullptr, ev_step = nullptr;
+  hipGraphExec_t gmain[MAX_SLOTS][2] = {};
+  // update lanes (moving bodies): streams and the events that order them
+  // against the step stream
+  hipStream_t lane_s[MAX_LANES] = {};
+  hipEvent_t ev_go = nullptr, ev_done[MAX_LANES] = {};
+  bool lanes_inflight = false;
+  // Timing (cc_sim_set_timing): event set e brackets the step graph
+  // (tev[e][0..1]) and each lane's updates (tev[e][2+2l..3+2l]) on the
+  // stream they run on.  Two sets alternate, so the host reads one block's events while
+  // the next block runs.
+  bool timing = false;
+  hipEvent_t tev[2][2 + 2 * MAX_LANES] = {};
+  bool tpending[2] = {false, false};
+  int tset = 0;
+  double t_verlet_ms = 0, t_update_ms = 0;
+  uint64_t t_steps = 0, t_updates = 0;
 
   ~Sim() {
     drop_graphs();
-    for (hipEvent_t e : {ev_fork, ev_body, ev_step})
-      if (e) (void)hipEventDestroy(e);
-    if (aux) (void)hipStreamDestroy(aux);
+    for (auto& set : tev)
+      for (hipEvent_t e : set)
+        if (e) (void)hipEventDestroy(e);
+    if (ev_go) (void)hipEventDestroy(ev_go);
+    for (int l = 0; l < MAX_LANES; ++l) {
+      if (ev_done[l]) (void)hipEventDestroy(ev_done[l]);
+      if (lane_s[l]) (void)hipStreamDestroy(lane_s[l]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
   void drop_graphs() {
-    for (auto* gs : {gexec, gmulti})
-      for (int k = 0; k < 2; ++k) {
-        if (gs[k]) (void)hipGraphExecDestroy(gs[k]);
-        gs[k] = nullptr;
+    for (auto& p : gmain)
+      for (hipGraphExec_t& g : p) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
       }
   }
 
@@ -251,10 +271,11 @@ struct Sim {
     return a;
   }
 
-  void launch_verlet(hipStream_t s) {
+  // the fused step reading body slot k and state[cur], writing state[cur ^ 1]
+  void launch_verlet(int k, hipStream_t s) {
     const float* in = state[cur].p;
     float* out = state[cur ^ 1].p;
-    const BvhView v = body.view(body.buf(cur));
+    const BvhView v = body.view(k);
     const unsigned g = grid_for(n, 256);
     if (debug && counting)
       k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
@@ -271,45 +292,26 @@ struct Sim {
     CC_HIP(hipGetLastError());
   }
 
-  // The body pose of frame f+1 (deformation, then refit or rebuild of the
-  // LBVH) depends only on f+1, not on the cloth.  So while the step of frame
-  // f reads body buffer buf(cur) on `stream`, the update of frame f+1 writes
-  // buffer buf(cur ^ 1) on `aux`:
-  //   step(k)   after update(k-1)   (it reads what update(k-1) wrote)
-  //   update(k) after step(k-1)     (it overwrites what step(k-1) read)
-  // Captured as one graph, the two chains become parallel branches joined by
-  // those edges; the graph ends with both joined.  Before a run buffer
-  // buf(cur) holds frame `frame`, and after it buf(cur) holds the new frame.
-  hipGraphExec_t capture(int count) {
-    const int cur0 = cur;
-    const bool pipe = body.moving();
-    hipGraph_t graph = nullptr;
-    CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    try {
-      if (pipe) {
-        CC_HIP(hipEventRecord(ev_fork, stream));
-        CC_HIP(hipStreamWaitEvent(aux, ev_fork, 0));
-      }
-      for (int k = 0; k < count; ++k) {
-        if (pipe && k > 0) CC_HIP(hipStreamWaitEvent(stream, ev_body, 0));
-        launch_verlet(stream);
-        if (pipe) {
-          if (k > 0) CC_HIP(hipStreamWaitEvent(aux, ev_step, 0));
-          body.update(cur ^ 1, aux);
-          CC_HIP(hipEventRecord(ev_step, stream));
-          CC_HIP(hipEventRecord(ev_body, aux));
-        }
-        cur ^= 1;
-      }
-      if (pipe) CC_HIP(hipStreamWaitEvent(stream, ev_body, 0));
-    } catch (...) {
-      cur = cur0;
-      hipGraph_t junk = nullptr;
-      (void)hipStreamEndCapture(stream, &junk);
-      if (junk) (void)hipGraphDestroy(junk);
-      throw;
+  // Direct (not captured) update of the body pose of frame g into its slot.
+  void update_direct(uint64_t g, hipStream_t s) {
+    body.update(body.slot_of(g), 0, nullptr, g, s);
+  }
+
+  // Moving body: drop the poses computed ahead and recompute the current
+  // frame's (after a change of frame, parameters or vertices).  Synchronous.
+  void refresh_body() {
+    if (body.moving()) {
+      update_direct(frame, stream);
+      ready = frame + 1;
+    } else if (body.mode == CC_BODY_STATIC) {
+      body.refit(0, 0, stream);
     }
+    CC_HIP(hipStreamSynchronize(stream));
+  }
+
+  hipGraphExec_t end_capture(int cur0) {
     cur = cur0;
+    hipGraph_t graph = nullptr;
     CC_HIP(hipStreamEndCapture(stream, &graph));
     hipGraphExec_t g = nullptr;
     hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
@@ -318,34 +320,139 @@ struct Sim {
     return g;
   }
 
-  // Recompute the body for the current frame into buf(cur) (after a change
-  // of frame, parameters or vertices).  Synchronous.
-  void refresh_body() {
-    if (body.mode == CC_BODY_NONE) return;
-    if (body.moving()) {
-      body.set_frame(frame, stream);
-      body.update(body.buf(cur), stream);
-    } else {
-      body.refit(0, stream);
+  template <class F>
+  hipGraphExec_t capture(F&& body_of_graph) {
+    const int cur0 = cur;
+    CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      body_of_graph();
+    } catch (...) {
+      cur = cur0;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
     }
-    CC_HIP(hipStreamSynchronize(stream));
+    return end_capture(cur0);
   }
 
-  // Steps are replayed from graphs of GRAPH_STEPS steps (one per starting
-  // parity; an even count keeps the parity) plus single-step graphs for the
-  // remainder: a graph boundary leaves the GPU idle for several
-  // microseconds, kernels inside a graph run back to back.
+  // GRAPH_STEPS fused steps from a frame in slot k0, starting at the current
+  // state parity.
+  hipGraphExec_t capture_main(int k0) {
+    return capture([&] {
+      for (int j = 0; j < GRAPH_STEPS; ++j) {
+        launch_verlet(body.slot_of((uint64_t)(k0 + j)), stream);
+        cur ^= 1;
+      }
+    });
+  }
+
+  // read the events of a finished timed block
+  void collect(int set) {
+    if (!tpending[set]) return;
+    hipEvent_t* te = tev[set];
+    const int G = GRAPH_STEPS;
+    CC_HIP(hipEventSynchronize(te[1]));
+    float ms = 0;
+    CC_HIP(hipEventElapsedTime(&ms, te[0], te[1]));
+    t_verlet_ms += ms;
+    t_steps += (uint64_t)G;
+    if (body.moving())
+      for (int l = 0; l < body.nlanes; ++l) {
+        CC_HIP(hipEventSynchronize(te[3 + 2 * l]));
+        CC_HIP(hipEventElapsedTime(&ms, te[2 + 2 * l], te[3 + 2 * l]));
+        t_update_ms += ms;
+        t_updates += (uint64_t)((G - l + body.nlanes - 1) / body.nlanes);
+      }
+    tpending[set] = false;
+  }
+
+  // the step stream waits for the update lanes of the last block
+  void join_lanes(hipStream_t s) {
+    if (!lanes_inflight) return;
+    for (int l = 0; l < body.nlanes; ++l) CC_HIP(hipStreamWaitEvent(s, ev_done[l], 0));
+    lanes_inflight = false;
+  }
+
+  void step_single(hipStream_t s) {
+    join_lanes(s);
+    launch_verlet(body.slot_of(frame), s);
+    cur ^= 1;
+    ++frame;
+    if (body.moving() && ready <= frame) {
+      update_direct(frame, s);
+      ready = frame + 1;
+    }
+  }
+
+  // One block of GRAPH_STEPS steps.  A moving body's poses for these frames
+  // are in their slots (the previous block's lanes computed them, or they
+  // are computed here first); the lanes compute the poses of the next
+  // GRAPH_STEPS frames, into the slots of the last GRAPH_STEPS frames (the
+  // ring holds 2 * GRAPH_STEPS), while the steps run:
+  //
+  //   step stream:  join(lanes of last block) -> go -> [step graph: 16 fused steps]
+  //   lane l:       wait(go) -> U(f+16+l), U(f+16+l+L), ... -> done[l]
+  //
+  // A dependency between streams costs several microseconds, so a block has
+  // two, not two per step.  "go" follows the previous step graph, so a lane
+  // overwrites a slot only after the steps that read it.
+  void step_graph(hipStream_t s) {
+    const int G = GRAPH_STEPS;
+    const int k0 = body.slot_of(frame);
+    hipEvent_t* te = timing ? tev[tset] : nullptr;
+    if (timing) collect(tset);
+    const bool pipe = body.moving();
+    if (pipe) {
+      join_lanes(s);
+      for (; ready < frame + G; ++ready) update_direct(ready, s);
+      CC_HIP(hipEventRecord(ev_go, s));
+    }
+    // The step graph is submitted first: submitting a graph to a stream that
+    // waits on another stream can hold the host until that wait is over,
+    // and the lanes' wait ("go") is over as soon as the step graph starts.
+    hipGraphExec_t& g = gmain[k0][cur];
+    if (!g) g = capture_main(k0);
+    if (te) CC_HIP(hipEventRecord(te[0], s));
+    CC_HIP(hipGraphLaunch(g, s));
+    if (te) CC_HIP(hipEventRecord(te[1], s));
+    if (pipe) {
+      // the lanes are launched directly: a graph launch can hold the host
+      // for most of the graph's run, which delays the next lane
+      for (int l = 0; l < body.nlanes; ++l) {
+        hipStream_t ls = lane_s[l];
+        CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
+        if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
+        for (int j = l; j < G; j += body.nlanes) {
+          const uint64_t f = frame + (uint64_t)(G + j);
+          body.update(body.slot_of(f), l, nullptr, f, ls);
+        }
+        if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
+        CC_HIP(hipEventRecord(ev_done[l], ls));
+      }
+      lanes_inflight = true;
+      ready = frame + 2 * G;
+    }
+    if (te) {
+      tpending[tset] = true;
+      tset ^= 1;
+    }
+    frame += (uint64_t)G;   // G is even: the state parity is unchanged
+  }
+
+  // Runs of GRAPH_STEPS steps go through the graphs; the rest are launched
+  // one by one, each followed by the body update of the next frame when it
+  // is not ready yet.
   void step(int nsteps, hipStream_t s) {
     if (!s) s = stream;
     while (nsteps > 0) {
-      const bool multi = nsteps >= GRAPH_STEPS;
-      hipGraphExec_t& g = multi ? gmulti[cur] : gexec[cur];
-      if (!g) g = capture(multi ? GRAPH_STEPS : 1);
-      CC_HIP(hipGraphLaunch(g, s));
-      const int k = multi ? GRAPH_STEPS : 1;
-      if (k & 1) cur ^= 1;
-      frame += (uint64_t)k;
-      nsteps -= k;
+      if (nsteps >= GRAPH_STEPS) {
+        step_graph(s);
+        nsteps -= GRAPH_STEPS;
+      } else {
+        step_single(s);
+        --nsteps;
+      }
     }
   }
 };
@@ -447,27 +554,39 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
                                        params->body_extend);
         if (rc) throw Error(rc, cc_last_error());
       }
-      MotionDesc md;
-      md.amp = bd->amp;
-      md.freq = bd->freq;
-      md.rot_amp = bd->rot_amp;
-      md.breath_amp = bd->breath_amp;
-      md.breath_freq = bd->breath_freq;
-      for (int c = 0; c < 3; ++c) {
-        md.dir[c] = bd->dir[c];
-        md.pivot[c] = bd->pivot[c];
-      }
-      S.body.dt = (double)params->dt;
-      S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, md, s);
-      // frame 0 into buffer 0; a refitted body keeps the topology built from
-      // the frame-0 pose in both buffers
-      S.body.deform(0, s);
-      S.body.build(0, s);
-      if (S.body.mode == CC_BODY_REFIT) S.body.mirror_topology(0, s);
+      MotionParams mp;
+      mp.amp = bd->amp;
+      mp.freq = bd->freq;
+      mp.rot_amp = bd->rot_amp;
+      mp.breath_amp = bd->breath_amp;
+      mp.breath_freq = bd->breath_freq;
+      mp.dx = bd->dir[0];
+      mp.dy = bd->dir[1];
+      mp.dz = bd->dir[2];
+      mp.px = bd->pivot[0];
+      mp.py = bd->pivot[1];
+      mp.pz = bd->pivot[2];
+      mp.dt = (double)params->dt;
+      int lanes = 2;
+      if (const char* e = std::getenv("CC_BODY_LANES")) lanes = std::atoi(e);
+      CC_REQUIRE(lanes >= 1 && lanes <= MAX_LANES, "cc_sim_create: CC_BODY_LANES must be 1..4");
+      S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, mp,
+                  2 * Sim::GRAPH_STEPS, lanes, s);
+      // frame 0 into slot 0; a refitted body keeps the topology built from
+      // the frame-0 pose in every slot
+      S.body.deform(0, nullptr, 0, s);
+      S.body.build(0, 0, s);
+      S.body.share_topology(0, s);
+      S.ready = 1;
       if (S.body.moving()) {
-        CC_HIP(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&S.ev_fork, &S.ev_body, &S.ev_step})
-          CC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        // Plain events: an event created with hipEventDisableTiming was seen
+        // to complete only with the command after it on its stream, which
+        // held the lanes back until the step graph had finished.
+        for (int l = 0; l < S.body.nlanes; ++l) {
+          CC_HIP(hipStreamCreateWithFlags(&S.lane_s[l], hipStreamNonBlocking));
+          CC_HIP(hipEventCreate(&S.ev_done[l]));
+        }
+        CC_HIP(hipEventCreate(&S.ev_go));
       }
     }
     CC_HIP(hipStreamSynchronize(s));
@@ -543,6 +662,7 @@ int cc_sim_set_frame(cc_sim* sim, uint64_t frame) {
     CC_REQUIRE(sim, "cc_sim_set_frame: null");
     CC_HIP(hipDeviceSynchronize());
     sim->s.frame = frame;
+    sim->s.ready = frame;
     sim->s.refresh_body();
   });
 }
@@ -582,7 +702,7 @@ int cc_sim_set_params(cc_sim* sim, const cc_params* p) {
     CC_HIP(hipDeviceSynchronize());
     sim->s.prm = *p;
     sim->s.body.depth = p->box_depth;
-    sim->s.body.dt = (double)p->dt;
+    sim->s.body.motion.dt = (double)p->dt;
     sim->s.refresh_body();
     sim->s.drop_graphs();
   });
@@ -663,17 +783,13 @@ int cc_body_set_vertices(cc_sim* sim, const float* verts3) {
     CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_set_vertices: no body");
     CC_HIP(hipDeviceSynchronize());
     Sim& S = sim->s;
-    const int b = B.buf(S.cur);
+    const int k = B.slot_of(S.frame);
     CC_HIP(hipMemcpy(B.rest.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
-    CC_HIP(hipMemcpy(B.verts[b].p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
-    // new rest pose: topology from it as given, then the current frame's pose
-    B.build(b, S.stream);
-    if (B.mode == CC_BODY_REFIT) B.mirror_topology(b, S.stream);
-    if (B.moving()) {
-      B.set_frame(S.frame, S.stream);
-      B.update(b, S.stream);
-    }
-    CC_HIP(hipStreamSynchronize(S.stream));
+    CC_HIP(hipMemcpy(B.slot[k].verts.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
+    // new rest pose: the topology from it as given, then the current frame's pose
+    B.build(k, 0, S.stream);
+    B.share_topology(k, S.stream);
+    S.refresh_body();
   });
 }
 
@@ -683,7 +799,7 @@ int cc_body_get_vertices(cc_sim* sim, float* verts3) {
     Body& B = sim->s.body;
     CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_get_vertices: no body");
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemcpy(verts3, B.verts[B.buf(sim->s.cur)].p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
+    CC_HIP(hipMemcpy(verts3, B.slot[B.slot_of(sim->s.frame)].verts.p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
   });
 }
 
@@ -694,7 +810,7 @@ int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath) {
     CC_REQUIRE(B.ncaps > 0, "cc_body_get_motion: body has no motion");
     CC_HIP(hipDeviceSynchronize());
     std::vector<float> m(12 + B.ncaps);
-    CC_HIP(hipMemcpy(m.data(), B.motion_cur.p, m.size() * 4, hipMemcpyDeviceToHost));
+    CC_HIP(hipMemcpy(m.data(), B.slot[B.slot_of(sim->s.frame)].motion.p, m.size() * 4, hipMemcpyDeviceToHost));
     if (R) std::memcpy(R, m.data(), 36);
     if (T) std::memcpy(T, m.data() + 9, 12);
     if (breath) std::memcpy(breath, m.data() + 12, 4 * (size_t)B.ncaps);
@@ -706,9 +822,10 @@ int cc_bvh_rebuild(cc_sim* sim) {
     CC_REQUIRE(sim, "cc_bvh_rebuild: null");
     CC_HIP(hipDeviceSynchronize());
     Sim& S = sim->s;
-    const int b = S.body.buf(S.cur);
-    S.body.build(b, S.stream);
-    if (S.body.mode == CC_BODY_REFIT) S.body.mirror_topology(b, S.stream);
+    const int k = S.body.slot_of(S.frame);
+    S.body.build(k, 0, S.stream);
+    S.body.share_topology(k, S.stream);
+    S.ready = S.frame + 1;    // poses computed ahead keep the old boxes
     CC_HIP(hipStreamSynchronize(S.stream));
   });
 }
@@ -717,7 +834,7 @@ int cc_bvh_refit(cc_sim* sim) {
   return guard([&] {
     CC_REQUIRE(sim, "cc_bvh_refit: null");
     CC_HIP(hipDeviceSynchronize());
-    sim->s.body.refit(sim->s.body.buf(sim->s.cur), sim->s.stream);
+    sim->s.body.refit(sim->s.body.slot_of(sim->s.frame), 0, sim->s.stream);
     CC_HIP(hipStreamSynchronize(sim->s.stream));
   });
 }
@@ -734,7 +851,12 @@ int cc_bvh_export(cc_sim* sim, uint64_t* keys, uint32_t* order, int32_t* childA,
   return guard([&] {
     CC_REQUIRE(sim, "cc_bvh_export: null");
     CC_HIP(hipDeviceSynchronize());
-    sim->s.body.export_tree(sim->s.body.buf(sim->s.cur), keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
+    Sim& S = sim->s;
+    const int k = S.body.slot_of(S.frame);
+    // a rebuilt body's keys and leaf parents live in lane scratch that later
+    // frames reuse: build this slot's tree again (same input, same tree)
+    if (S.body.mode == CC_BODY_REBUILD) S.body.build(k, 0, S.stream);
+    sim->s.body.export_tree(k, keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
                             sim->s.stream);
   });
 }
@@ -755,7 +877,7 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     on.alloc(3 * (size_t)n);
     h.alloc(n);
     in.alloc(n);
-    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.buf(S.cur)), p.p, n, h.p, in.p, op.p, on.p);
+    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.slot_of(S.frame)), p.p, n, h.p, in.p, op.p, on.p);
     CC_HIP(hipGetLastError());
     CC_HIP(hipMemcpyAsync(hit, h.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
@@ -831,17 +953,18 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     std::vector<hipEvent_t> ev(5);
     for (auto& e : ev) CC_HIP(hipEventCreate(&e));
     double t_body = 0, t_upd = 0, t_ver = 0;
-    // the same work as a step, one kernel group at a time: the body update of
-    // the next frame into the other buffer, then the step of this frame
+    // the work of a step one kernel group at a time: the body update of the
+    // next frame, then the step of this frame
+    S.ready = S.frame + 1;
     for (int it = 0; it < nsteps; ++it) {
-      const int nb = S.body.buf(S.cur ^ 1);
+      const int nk = S.body.slot_of(S.frame + 1);
       CC_HIP(hipEventRecord(ev[0], s));
-      if (S.body.moving()) S.body.deform(nb, s);
+      if (S.body.moving()) S.body.deform(nk, nullptr, S.frame + 1, s);
       CC_HIP(hipEventRecord(ev[1], s));
-      if (S.body.mode == CC_BODY_REFIT) S.body.refit(nb, s);
-      else if (S.body.mode == CC_BODY_REBUILD) S.body.build(nb, s);
+      if (S.body.mode == CC_BODY_REFIT) S.body.refit(nk, 0, s);
+      else if (S.body.mode == CC_BODY_REBUILD) S.body.build(nk, 0, s);
       CC_HIP(hipEventRecord(ev[2], s));
-      S.launch_verlet(s);
+      S.launch_verlet(S.body.slot_of(S.frame), s);
       CC_HIP(hipEventRecord(ev[3], s));
       CC_HIP(hipEventSynchronize(ev[3]));
       float a, b, c;
@@ -853,6 +976,7 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
       t_ver += c;
       S.cur ^= 1;
       ++S.frame;
+      S.ready = S.frame + 1;
     }
     // whole steps from the captured graphs
     CC_HIP(hipEventRecord(ev[0], s));
@@ -868,6 +992,38 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     else out->build_us = 1e3 * t_upd / nsteps;
     out->verlet_us = 1e3 * t_ver / nsteps;
     out->step_us = 1e3 * w / nsteps;
+  });
+}
+
+int cc_sim_set_timing(cc_sim* sim, int enable) {
+  return guard([&] {
+    CC_REQUIRE(sim, "cc_sim_set_timing: null");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    for (int e = 0; e < 2; ++e) S.collect(e);
+    S.timing = enable != 0;
+    if (S.timing && !S.tev[0][0])
+      for (auto& set : S.tev)
+        for (hipEvent_t& ev : set) CC_HIP(hipEventCreate(&ev));
+    S.t_verlet_ms = S.t_update_ms = 0;
+    S.t_steps = S.t_updates = 0;
+    S.tpending[0] = S.tpending[1] = false;
+  });
+}
+
+int cc_sim_get_timing(cc_sim* sim, cc_timing* out) {
+  return guard([&] {
+    CC_REQUIRE(sim && out, "cc_sim_get_timing: null");
+    Sim& S = sim->s;
+    for (int e = 0; e < 2; ++e) S.collect(e);
+    std::memset(out, 0, sizeof(*out));
+    out->steps = (uint32_t)S.t_steps;
+    if (S.t_steps) out->verlet_us = 1e3 * S.t_verlet_ms / (double)S.t_steps;
+    if (S.t_updates) {
+      const double u = 1e3 * S.t_update_ms / (double)S.t_updates;
+      if (S.body.mode == CC_BODY_REBUILD) out->build_us = u;
+      else out->refit_us = u;
+    }
   });
 }
 

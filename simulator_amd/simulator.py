@@ -279,8 +279,19 @@ class Simulator:
         return nrm
 
     def time_kernels(self, nsteps):
+        """Per-kernel-group times with nothing overlapped (cc_sim_time_kernels)."""
         t = L.cc_timing()
         L.check(L.lib().cc_sim_time_kernels(self._h, int(nsteps), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in L.cc_timing._fields_}
+
+    def set_timing(self, enable=True):
+        """Event timing inside the step graphs (cc_sim_set_timing)."""
+        L.check(L.lib().cc_sim_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """Averages of the in-graph event timing since it was enabled."""
+        t = L.cc_timing()
+        L.check(L.lib().cc_sim_get_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in L.cc_timing._fields_}
 
 
