@@ -8,61 +8,103 @@
 
 namespace cc {
 
-// D_BVH::intersect (cudaBRT.cu:239-285).  At each internal node both child
+// Deferred-node stack of one traversal beyond the four entries held in
+// registers: entry k of a lane is p[k * stride] (lanes of a wave adjacent,
+// so spills coalesce).  It lives in global memory that the caller provides,
+// STACK_SPILL entries per lane: a private array here would make the kernel
+// use scratch memory, which was seen to cap the number of resident waves.
+constexpr int STACK_SPILL = 60;
+struct Spill {
+  int32_t* p;
+  uint32_t stride;
+};
+
+// Deferred nodes: the top four in registers (static shifts, no indexed
+// private memory), older ones in the spill area.
+struct TravStack {
+  int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int sp = 0;
+  Spill spill;
+  __device__ bool push(int32_t v) {
+    if (sp == 64) return false;          // unreachable for depth < 64
+    if (sp >= 4) spill.p[(size_t)(sp - 4) * spill.stride] = s3;
+    s3 = s2;
+    s2 = s1;
+    s1 = s0;
+    s0 = v;
+    ++sp;
+    return true;
+  }
+  __device__ bool pop(int32_t& v) {
+    if (sp == 0) return false;
+    v = s0;
+    s0 = s1;
+    s1 = s2;
+    s2 = s3;
+    --sp;
+    if (sp >= 4) s3 = spill.p[(size_t)(sp - 4) * spill.stride];
+    return true;
+  }
+};
+
+enum : int { STEP_DESCEND = -1, STEP_POP = -2, STEP_MISS = -3 };
+
+// One internal node of D_BVH::intersect (cudaBRT.cu:239-285), given its
+// record (q0..q2 = child boxes A and B, L = childA, childB, ...).  Both child
 // boxes are tested; an overlapped LEAF child is returned at once (A before
 // B); otherwise overlapped internal children are descended left first with
-// the right one deferred on a per-lane stack.  The stack never holds more
-// entries than the node depth, and the depth of a Karras tree over unique
-// 64-bit keys is below 64, so 64 entries cannot overflow.
-// Returns the sorted leaf index, or -1.  *visits counts internal nodes.
-__device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits) {
+// the right one deferred on the stack.  Returns the leaf (>= 0), or
+// STEP_DESCEND / STEP_POP with the next node in `next`, or STEP_MISS.
+// Both overlap tests and the leaf decision are computed before the first
+// branch (`&` and a select, not `&&` and two returns): otherwise the
+// compiler moves the loads of the second box behind the first test, and a
+// node costs two memory round trips instead of one.
+__device__ inline int32_t trav_step(f3 p, float4 q0, float4 q1, float4 q2, int4 L, TravStack& st,
+                                    int32_t& next) {
+  // A = [q0.xyz, (q0.w, q1.x, q1.y)], B = [(q1.z, q1.w, q2.x), q2.yzw]
+  const bool oL = (p.x >= q0.x) & (p.x <= q0.w) & (p.y >= q0.y) & (p.y <= q1.x) & (p.z >= q0.z) &
+                  (p.z <= q1.y);
+  const bool oR = (p.x >= q1.z) & (p.x <= q2.y) & (p.y >= q1.w) & (p.y <= q2.z) & (p.z >= q2.x) &
+                  (p.z <= q2.w);
+  const int32_t ca = L.x, cb = L.y;
+  const bool hitA = oL & (ca < 0), hitB = oR & (cb < 0);
+  const int32_t leaf = hitA ? ~ca : ~cb;   // A before B
+  if (hitA | hitB) return leaf;
+  const bool tL = oL & (ca >= 0), tR = oR & (cb >= 0);
+  if (tL) {
+    if (tR && !st.push(cb)) return STEP_MISS;
+    next = ca;
+    return STEP_DESCEND;
+  }
+  if (tR) {
+    next = cb;
+    return STEP_DESCEND;
+  }
+  return st.pop(next) ? STEP_POP : STEP_MISS;
+}
+
+// First-hit traversal (D_BVH::intersect, cudaBRT.cu:239-285): one node
+// record (64 bytes, both child boxes and links) per memory round trip.  The
+// stack never holds more entries than the node depth, and the depth of a
+// Karras tree over unique 64-bit keys is below 64, so 64 entries cannot
+// overflow.  Returns the sorted leaf index, or -1.  *visits counts internal
+// nodes processed.
+__device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill) {
   if (b.m == 0) return -1;
   if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
-  // Short stack: the top four entries live in registers (static shifts, no
-  // indexed private memory); older entries spill to a private array that is
-  // touched only by paths deeper than four deferred nodes.
-  int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  int32_t spill[60];
-  int sp = 0;
+  TravStack st;
+  st.spill = spill;
   int32_t node = 0;
   // A well-formed tree visits each internal node at most once; the bound
   // only guards against a corrupted tree (the loop always terminates).
   for (uint32_t guard = 0; guard < b.m; ++guard) {
-    ++visits;
     const float4* nd = reinterpret_cast<const float4*>(b.nodes + node);
     const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2];
     const int4 L = reinterpret_cast<const int4*>(nd)[3];
-    // A = [q0.xyz, (q0.w, q1.x, q1.y)], B = [(q1.z, q1.w, q2.x), q2.yzw]
-    const bool oL = (p.x >= q0.x && p.x <= q0.w) && (p.y >= q0.y && p.y <= q1.x) &&
-                    (p.z >= q0.z && p.z <= q1.y);
-    const bool oR = (p.x >= q1.z && p.x <= q2.y) && (p.y >= q1.w && p.y <= q2.z) &&
-                    (p.z >= q2.x && p.z <= q2.w);
-    const int32_t ca = L.x, cb = L.y;
-    if (oL && ca < 0) return ~ca;
-    if (oR && cb < 0) return ~cb;
-    const bool tL = oL && ca >= 0, tR = oR && cb >= 0;
-    if (tL) {
-      if (tR) {                           // push cb
-        if (sp == 64) return -1;          // unreachable for depth < 64
-        if (sp >= 4) spill[sp - 4] = s3;
-        s3 = s2;
-        s2 = s1;
-        s1 = s0;
-        s0 = cb;
-        ++sp;
-      }
-      node = ca;
-    } else if (tR) {
-      node = cb;
-    } else {                              // pop
-      if (sp == 0) return -1;
-      node = s0;
-      s0 = s1;
-      s1 = s2;
-      s2 = s3;
-      --sp;
-      if (sp >= 4) s3 = spill[sp - 4];
-    }
+    ++visits;
+    const int32_t r = trav_step(p, q0, q1, q2, L, st, node);
+    if (r >= 0) return r;
+    if (r == STEP_MISS) return -1;
   }
   return -1;
 }
@@ -81,8 +123,8 @@ __device__ inline bool plane_test(const float* verts, uint4 t, f3 p, float& d, f
 // first-hit leaf's plane test says inside, p += |d| n and the contact normal
 // is n, else the contact normal is 0.  Returns true when projected.
 __device__ inline bool collide_project(const BvhView& b, f3& p, f3& n_out, int32_t& hit,
-                                       uint32_t& visits) {
-  hit = bvh_first_hit(b, p, visits);
+                                       uint32_t& visits, Spill spill) {
+  hit = bvh_first_hit(b, p, visits, spill);
   n_out = mk(0.0f, 0.0f, 0.0f);
   if (hit < 0) return false;
   float d;

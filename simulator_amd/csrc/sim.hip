@@ -38,17 +38,24 @@ template <bool DEBUG, bool COUNT>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
                                                 float* __restrict__ contact, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
-                                                uint32_t n, float* __restrict__ dbg_pre,
+                                                uint32_t n, int32_t* __restrict__ spill, float* __restrict__ dbg_pre,
                                                 int32_t* __restrict__ dbg_hit, uint8_t* __restrict__ dbg_in,
                                                 unsigned long long* __restrict__ ctr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t visits = 0;
-  bool active = i < n;
+  const bool active = i < n;
   if (active) {
-    const uint32_t r0raw = row[i], r1 = row[i + 1] & ROW_MASK;
-    const uint32_t r0 = r0raw & ROW_MASK;
-    const float* s = sin_ + 6 * (size_t)i;
-    const f3 p = mk(s[0], s[1], s[2]), q = mk(s[3], s[4], s[5]);
+    // The vertex's own records are loaded together and waited for once: the
+    // compiler otherwise issues each load at its first use, one memory round
+    // trip after another.
+    const uint32_t r0raw = row[i], r1raw = row[i + 1];
+    const float2* s2 = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)i);
+    const float2 e0 = s2[0], e1 = s2[1], e2 = s2[2];
+    const f3 c = ld3(contact + 3 * (size_t)i);
+    asm volatile("" ::"v"(r0raw), "v"(r1raw), "v"(e0.x), "v"(e0.y), "v"(e1.x), "v"(e1.y), "v"(e2.x), "v"(e2.y),
+                 "v"(c.x), "v"(c.y), "v"(c.z));
+    const uint32_t r0 = r0raw & ROW_MASK, r1 = r1raw & ROW_MASK;
+    const f3 p = mk(e0.x, e0.y, e1.x), q = mk(e1.y, e2.x, e2.y);
     f3 np, nq, nc = mk(0.0f, 0.0f, 0.0f);
     int32_t hit = -1;
     bool inside = false;
@@ -61,12 +68,19 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
       f3 S = mk(0.f, 0.f, 0.f);   // all spring types, in merged-row order
       // batches of SPRING_BATCH springs: all record loads, then all
       // neighbour gathers are in flight together
+      // Batches of SPRING_BATCH springs, branch-free: all record loads, then
+      // all neighbour gathers, are issued before any use.  Slots past the
+      // row end gather the vertex itself and contribute exactly +0 (with a
+      // condition per slot the compiler sinks each gather into its branch,
+      // one memory round trip per spring).
       for (uint32_t k0 = r0; k0 < r1; k0 += SPRING_BATCH) {
         uint2 rr[SPRING_BATCH];
         float2 g[SPRING_BATCH][3];
 #pragma unroll
-        for (int q = 0; q < SPRING_BATCH; ++q)
-          rr[q] = (k0 + q < r1) ? rec[k0 + q] : make_uint2(i, 0u);
+        for (int q = 0; q < SPRING_BATCH; ++q) {
+          rr[q] = rec[min(k0 + (uint32_t)q, r1 - 1)];
+          if (k0 + q >= r1) rr[q] = make_uint2(i, 0u);
+        }
 #pragma unroll
         for (int q = 0; q < SPRING_BATCH; ++q) {
           const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(rr[q].x & END_MASK));
@@ -76,7 +90,6 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
         }
 #pragma unroll
         for (int q = 0; q < SPRING_BATCH; ++q) {
-          if (k0 + q >= r1) break;
           const uint32_t t = rr[q].x >> 30;
           const float rest = __uint_as_float(rr[q].y) * a.rest_scale;
           const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
@@ -88,14 +101,15 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
           const float rinv = rsqrtf(L2);
           const float L = L2 * rinv;
           const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
-          const float sc = fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv;
+          // an empty slot: dP = (eps, eps, eps) > 0 and sc = +0, so the
+          // products are +0 and S (never -0) is unchanged
+          const float sc = (k0 + q < r1) ? fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv : 0.0f;
           S.x = fmaf(dP.x, sc, S.x);
           S.y = fmaf(dP.y, sc, S.y);
           S.z = fmaf(dP.z, sc, S.z);
         }
       }
       F = F + S;
-      const f3 c = ld3(contact + 3 * (size_t)i);
       const float cf = fmaf(c.z, F.z, fmaf(c.y, F.y, c.x * F.x));
       F = mk(fmaf(-c.x, cf, F.x), fmaf(-c.y, cf, F.y), fmaf(-c.z, cf, F.z));
       // pos + pos - pos_old + acc*dt*dt (verlet.cu:155) evaluated as
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
               p.z + fmaf(F.z * a.inv_mass, a.dt2, p.z - q.z));
       nq = p;
       if (DEBUG) st3(dbg_pre + 3 * (size_t)i, np);
-      inside = collide_project(bvh, np, nc, hit, visits);
+      inside = collide_project(bvh, np, nc, hit, visits, Spill{spill + i, n});
       if (inside) nq = np;
     }
     if (DEBUG) {
@@ -139,14 +153,15 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
 }
 
 __global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float* __restrict__ pts, uint32_t n,
-                                                        int32_t* __restrict__ hit, uint8_t* __restrict__ ins,
-                                                        float* __restrict__ out_pos, float* __restrict__ out_n) {
+                                                        int32_t* __restrict__ spill, int32_t* __restrict__ hit,
+                                                        uint8_t* __restrict__ ins, float* __restrict__ out_pos,
+                                                        float* __restrict__ out_n) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   f3 p = ld3(pts + 3 * (size_t)i), nc;
   int32_t h;
   uint32_t visits = 0;
-  const bool in = collide_project(bvh, p, nc, h, visits);
+  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n});
   hit[i] = h;
   ins[i] = in ? 1 : 0;
   st3(out_pos + 3 * (size_t)i, p);
@@ -196,6 +211,7 @@ struct Sim {
   DBuf<float> state[2];
   DBuf<float> contact;
   DBuf<uint32_t> row;
+  DBuf<int32_t> spill;      // [STACK_SPILL * n] traversal stack spill of the step kernel
   DBuf<uint2> rec;
   std::vector<uint32_t> h_row;
   int cur = 0;              // state[cur] holds frame `frame`
@@ -278,16 +294,16 @@ struct Sim {
     const BvhView v = body.view(k);
     const unsigned g = grid_for(n, 256);
     if (debug && counting)
-      k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+      k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, spill.p, dbg_pre.p, dbg_hit.p,
                                              dbg_in.p, ctr.p);
     else if (debug)
-      k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+      k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, spill.p, dbg_pre.p, dbg_hit.p,
                                               dbg_in.p, ctr.p);
     else if (counting)
-      k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p, dbg_hit.p,
+      k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, spill.p, dbg_pre.p, dbg_hit.p,
                                               dbg_in.p, ctr.p);
     else
-      k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, dbg_pre.p,
+      k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, args(), v, n, spill.p, dbg_pre.p,
                                                dbg_hit.p, dbg_in.p, ctr.p);
     CC_HIP(hipGetLastError());
   }
@@ -536,6 +552,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         packed[6 * (size_t)i + c] = x3[3 * (size_t)i + c];
         packed[6 * (size_t)i + 3 + c] = x3[3 * (size_t)i + c];
       }
+    S.spill.alloc((size_t)STACK_SPILL * S.n);
     S.state[0].upload(packed.data(), packed.size(), s);
     S.state[1].alloc(packed.size());
     S.state[1].zero(s);
@@ -872,12 +889,14 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     DBuf<float> p, op, on;
     DBuf<int32_t> h;
     DBuf<uint8_t> in;
+    DBuf<int32_t> sp;
     p.upload(pts3, 3 * (size_t)n, s);
+    sp.alloc((size_t)STACK_SPILL * n);
     op.alloc(3 * (size_t)n);
     on.alloc(3 * (size_t)n);
     h.alloc(n);
     in.alloc(n);
-    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.slot_of(S.frame)), p.p, n, h.p, in.p, op.p, on.p);
+    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.slot_of(S.frame)), p.p, n, sp.p, h.p, in.p, op.p, on.p);
     CC_HIP(hipGetLastError());
     CC_HIP(hipMemcpyAsync(hit, h.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
