@@ -23,29 +23,16 @@ struct MotionParams {
   uint32_t ncaps = 0;
 };
 
-// The parameters of frame `frame`: out[0..8] = R (row-major rotation about
-// +y), out[9..11] = T (with the pivot folded in), out[12+k] = breathing
-// factor of capsule k.  Computed in double precision with one rounding to
-// float; threads tid, tid+nthreads, ... of one block share the work.
-__device__ inline void compute_motion(const MotionParams& a, const float* __restrict__ phase, uint64_t frame,
-                                      float* out, uint32_t tid, uint32_t nthreads) {
-  const double PI2 = 6.283185307179586476925;
-  const double t = (double)frame * a.dt;
-  if (tid == 0) {
-    const double w = sin(PI2 * a.freq * t);
-    const double th = a.rot_amp * w, c = cos(th), s = sin(th);
-    const double R[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
-    const double P[3] = {a.px, a.py, a.pz};
-    const double D[3] = {a.dx, a.dy, a.dz};
-    for (int r = 0; r < 3; ++r) {
-      const double rp = R[3 * r] * P[0] + R[3 * r + 1] * P[1] + R[3 * r + 2] * P[2];
-      out[9 + r] = (float)(P[r] - rp + a.amp * w * D[r]);
-    }
-    for (int q = 0; q < 9; ++q) out[q] = (float)R[q];
-  }
-  for (uint32_t k = tid; k < a.ncaps; k += nthreads)
-    out[12 + k] = (float)(1.0 + a.breath_amp * sin(PI2 * a.breath_freq * t + (double)phase[k]));
-}
+// The parameters of one frame, by value: v[0..8] = R (row-major rotation
+// about +y), v[9..11] = T (with the pivot folded in), v[12+k] = breathing
+// factor of capsule k.
+struct MotionFrame {
+  float v[12 + MAX_CAPS];
+};
+
+// Motion of frame `frame` (t = frame * dt), computed on the host in double
+// precision with one rounding to float.
+MotionFrame motion_of(const MotionParams& a, const float* phase_host, uint64_t frame);
 
 // One frame of the body as the collision query reads it.
 struct Slot {
@@ -101,6 +88,7 @@ struct Body {
   uint32_t ncaps = 0;
   DBuf<uint32_t> cap_vstart;        // [ncaps+1]
   DBuf<float> cap_p0, cap_p1, cap_phase;
+  std::vector<float> h_phase;       // [ncaps] host copy for the motion
   MotionParams motion;
 
   int nslots = 1, nlanes = 1;
@@ -131,14 +119,12 @@ struct Body {
   void init(const float* verts3, uint32_t nv, const uint32_t* tris, uint32_t nf, uint32_t mode, float depth,
             const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s);
 
-  // Launches, asynchronous on s.  The frame of a deformation is
-  // (*base if base else 0) + off: graphs read a device frame counter, direct
-  // calls pass the frame itself.
-  void deform(int k, const uint64_t* base, uint64_t off, hipStream_t s);   // rest -> slot k pose
+  // launches, asynchronous on s
+  void deform(int k, uint64_t frame, hipStream_t s);   // rest -> slot k pose of `frame`
   void build(int k, int l, hipStream_t s);      // full LBVH build of slot k on lane l (includes the refit)
   void top_setup(int k, int l, hipStream_t s);  // refit description of slot k's topology into lane l
   void refit(int k, int l, hipStream_t s);      // boxes of slot k
-  void update(int k, int l, const uint64_t* base, uint64_t off, hipStream_t s);   // per-mode frame update
+  void update(int k, int l, uint64_t frame, hipStream_t s);   // per-mode update of slot k to `frame`
   void share_topology(int from, hipStream_t s); // copy slot `from`'s tree to every other slot
 
   // host export in the reference encoding (lane 0 holds the keys and leaf

@@ -9,6 +9,7 @@
 // stage stays on the device; keys are (morton << 32 | primitive) so all
 // primitives are kept and the order is total.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "cc_body.h"
@@ -16,30 +17,44 @@
 namespace cc {
 
 // ------------------------------------------------------------ deformation
-// Every block computes the frame's motion parameters into LDS (block 0 also
-// stores them for the host), then every thread moves one vertex: radial
-// breathing about the closest point of its capsule axis, then R * Y + T.
-// The frame is (*base if base else 0) + off.
+MotionFrame motion_of(const MotionParams& a, const float* phase, uint64_t frame) {
+  const double PI2 = 6.283185307179586476925;
+  const double t = (double)frame * a.dt;
+  MotionFrame m{};
+  const double w = std::sin(PI2 * a.freq * t);
+  const double th = a.rot_amp * w, c = std::cos(th), s = std::sin(th);
+  const double R[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
+  const double P[3] = {a.px, a.py, a.pz};
+  const double D[3] = {a.dx, a.dy, a.dz};
+  for (int r = 0; r < 3; ++r) {
+    const double rp = R[3 * r] * P[0] + R[3 * r + 1] * P[1] + R[3 * r + 2] * P[2];
+    m.v[9 + r] = (float)(P[r] - rp + a.amp * w * D[r]);
+  }
+  for (int q = 0; q < 9; ++q) m.v[q] = (float)R[q];
+  for (uint32_t k = 0; k < a.ncaps; ++k)
+    m.v[12 + k] = (float)(1.0 + a.breath_amp * std::sin(PI2 * a.breath_freq * t + (double)phase[k]));
+  return m;
+}
+
+// Every thread moves one vertex: radial breathing about the closest point of
+// its capsule axis, then R * Y + T; block 0 also stores the frame's motion
+// parameters for the host.
 __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, float* __restrict__ out,
-                                                uint32_t nv, const uint32_t* __restrict__ vstart,
+                                                uint32_t nv, uint32_t ncaps, const uint32_t* __restrict__ vstart,
                                                 const float* __restrict__ p0, const float* __restrict__ p1,
-                                                const float* __restrict__ phase, MotionParams mp,
-                                                const uint64_t* __restrict__ base, uint64_t off,
-                                                float* __restrict__ motion_out) {
-  __shared__ float sP[12 + MAX_CAPS], sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
+                                                MotionFrame mf, float* __restrict__ motion_out) {
+  __shared__ float sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
   __shared__ uint32_t sV[MAX_CAPS + 1];
   const uint32_t tid = threadIdx.x;
-  const uint32_t ncaps = mp.ncaps;
-  const uint64_t frame = (base ? *base : 0ull) + off;
-  compute_motion(mp, phase, frame, sP, tid, blockDim.x);
+  const float* sP = mf.v;
   for (uint32_t k = tid; k < 3 * ncaps; k += blockDim.x) {
     sP0[k] = p0[k];
     sP1[k] = p1[k];
   }
   for (uint32_t k = tid; k <= ncaps; k += blockDim.x) sV[k] = vstart[k];
-  __syncthreads();
   if (blockIdx.x == 0)
     for (uint32_t k = tid; k < 12 + ncaps; k += blockDim.x) motion_out[k] = sP[k];
+  __syncthreads();
   const uint32_t v = blockIdx.x * blockDim.x + tid;
   if (v >= nv) return;
   // capsule of v: last k with vstart[k] <= v
@@ -312,6 +327,7 @@ void Lane::alloc(uint32_t m, bool full_lane, hipStream_t s) {
   hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
   keys.alloc(m);
   leaf_parent.alloc(m);
+  leaf_parent.zero(s);
   top_list.alloc(mi);
   top_pos.alloc(mi);
   top_count.alloc(1);
@@ -358,6 +374,7 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
     cap_p0.upload(caps->p0, 3 * (size_t)ncaps, s);
     cap_p1.upload(caps->p1, 3 * (size_t)ncaps, s);
     cap_phase.upload(caps->phase, ncaps, s);
+    h_phase.assign(caps->phase, caps->phase + ncaps);
   }
   const size_t mi = std::max<uint32_t>(m, 2) - 1;
   if (mode != CC_BODY_REBUILD) leaf_tri.alloc(m);
@@ -376,10 +393,11 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   for (int l = 0; l < nlanes; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, s);
 }
 
-void Body::deform(int k, const uint64_t* base, uint64_t off, hipStream_t s) {
+void Body::deform(int k, uint64_t frame, hipStream_t s) {
   if (nv == 0 || ncaps == 0) return;
-  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, slot[k].verts.p, nv, cap_vstart.p, cap_p0.p, cap_p1.p,
-                                               cap_phase.p, motion, base, off, slot[k].motion.p);
+  const MotionFrame mf = motion_of(motion, h_phase.data(), frame);
+  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, slot[k].verts.p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
+                                               mf, slot[k].motion.p);
   CC_HIP(hipGetLastError());
 }
 
@@ -427,12 +445,12 @@ void Body::share_topology(int from, hipStream_t s) {
   }
 }
 
-void Body::update(int k, int l, const uint64_t* base, uint64_t off, hipStream_t s) {
+void Body::update(int k, int l, uint64_t frame, hipStream_t s) {
   if (mode == CC_BODY_REFIT) {
-    deform(k, base, off, s);
+    deform(k, frame, s);
     refit(k, l, s);
   } else if (mode == CC_BODY_REBUILD) {
-    deform(k, base, off, s);
+    deform(k, frame, s);
     build(k, l, s);
   }
 }

@@ -29,7 +29,8 @@ constexpr int RC_CHUNK = 2048;
 constexpr int RC_THREADS = 512;
 constexpr int RC_PER_THREAD = RC_CHUNK / RC_THREADS;
 constexpr int TOP_THREADS = 1024;
-constexpr int TOP_REFIT_THREADS = 256;
+constexpr int TOP_REFIT_THREADS = 1024;
+constexpr int TOP_LOAD = 4;                                         // entries per thread per load round
 constexpr int TOP_MAX_LDS = 4096;                                   // entries held in LDS (36 B each)
 constexpr int TOP_MAX_LEVELS = 64;                                  // top-tree height bound (depth < 64)
 constexpr uint8_t BELOW_TOP = 0x80;                                 // flag: parent is a top node
@@ -223,30 +224,51 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
   const uint32_t tid = threadIdx.x;
   const uint32_t H = m > 1 ? chunk_h[blockIdx.x] : 0;
 
-  // leaf boxes: the loads of all of the thread's leaves first
+  // Loads in two rounds, each issued in full before its first use (indices
+  // clamped instead of guarded: a guard per entry lets the compiler sink
+  // each load into its branch, one memory round trip per entry).
+  // Round 1: the thread's leaves (triangle, flags, parent) and internal
+  // nodes (links, flags).
+  const int32_t lmax = c1 - 1, xmax = max(i1 - 1, 0);
+  const int32_t tmax = max((int32_t)m - 2, 0);                  // last entry of top_pos
   uint4 tri[RC_PER_THREAD];
+  int4 L[RC_PER_THREAD];
+  uint8_t fl[RC_PER_THREAD], lf[RC_PER_THREAD];
+  int32_t lpar[RC_PER_THREAD];
+#pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
-    tri[r] = l < c1 ? leaf_tri[l] : make_uint4(0, 0, 0, 0);
+    const int32_t lc = min(l, lmax), xc = min(l, xmax);
+    tri[r] = leaf_tri[lc];
+    lf[r] = lflag[lc];
+    lpar[r] = leaf_parent[lc];
+    L[r] = reinterpret_cast<const int4*>(nodes + xc)[3];
+    fl[r] = nflag[xc];
   }
-  // topology of the thread's internal nodes
-  int4 L[RC_PER_THREAD];
-  uint8_t fl[RC_PER_THREAD];
+#pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const int32_t x = c0 + r * RC_THREADS + (int32_t)tid;
-    L[r] = make_int4(0, 0, -1, 0);
-    fl[r] = 0;
-    if (x < i1) {
-      L[r] = reinterpret_cast<const int4*>(nodes + x)[3];
-      fl[r] = nflag[x];
+    if (x >= i1) {
+      L[r] = make_int4(0, 0, -1, 0);
+      fl[r] = 0;
     }
+    if (x >= c1) lf[r] = 0;
   }
+  // Round 2: the leaves' vertices, and the top-pass record of every parent
+  // that a box goes to directly (the parent's top position).
   float vtx[RC_PER_THREAD][9];
+  int32_t tpos_n[RC_PER_THREAD], tpos_l[RC_PER_THREAD];
+#pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const uint32_t t3[3] = {tri[r].x, tri[r].y, tri[r].z};
+#pragma unroll
     for (int k = 0; k < 3; ++k)
+#pragma unroll
       for (int c = 0; c < 3; ++c) vtx[r][3 * k + c] = verts[3 * (size_t)t3[k] + c];
+    tpos_n[r] = top_pos[min(max(L[r].z, 0), tmax)];
+    tpos_l[r] = top_pos[min(max(lpar[r], 0), tmax)];
   }
+#pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
     if (l >= c1) continue;
@@ -295,19 +317,16 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
         } else {
           const bool isA = (fl[r] & IS_A) != 0;
           put6(isA ? nodes[p].a_min : nodes[p].b_min, bx, bx + 3);
-          put6(top_child + 12 * (size_t)top_pos[p] + (isA ? 0 : 6), bx, bx + 3);
+          put6(top_child + 12 * (size_t)tpos_n[r] + (isA ? 0 : 6), bx, bx + 3);
         }
       }
     }
     const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
-    if (l < c1) {
-      const uint8_t lf = lflag[l];
-      if (lf & BELOW_TOP) {
-        const int32_t p = leaf_parent[l];
-        const bool isA = (lf & IS_A) != 0;
-        put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
-        put6(top_child + 12 * (size_t)top_pos[p] + (isA ? 0 : 6), lbox[l - c0], lbox[l - c0] + 3);
-      }
+    if (l < c1 && (lf[r] & BELOW_TOP)) {
+      const int32_t p = lpar[r];
+      const bool isA = (lf[r] & IS_A) != 0;
+      put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
+      put6(top_child + 12 * (size_t)tpos_l[r] + (isA ? 0 : 6), lbox[l - c0], lbox[l - c0] + 3);
     }
   }
 }
@@ -371,15 +390,34 @@ __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t 
                                  const int4* __restrict__ links, const float* __restrict__ top_child,
                                  float* __restrict__ root_out, float* box, int2* lnk, int32_t* lsl) {
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < T; i += TOP_REFIT_THREADS) {
-    const int4 L = links[i];
-    const float* tc = top_child + 12 * (size_t)i;
-    float mn[3] = {100.0f, 100.0f, 100.0f}, mx[3] = {-100.0f, -100.0f, -100.0f};
-    if (L.x < 0) unite(mn, mx, tc);
-    if (L.y < 0) unite(mn, mx, tc + 6);
-    put6(box + 6 * (size_t)i, mn, mx);
-    lnk[i] = make_int2(L.x, L.y);
-    lsl[i] = L.w;
+  // Entry boxes from the non-top children's boxes.  TOP_LOAD entries per
+  // thread per round, every load of a round issued before any use (indices
+  // clamped, records of top children loaded and then ignored).
+  for (uint32_t i0 = 0; i0 < T; i0 += TOP_REFIT_THREADS * TOP_LOAD) {
+    int4 L[TOP_LOAD];
+    float4 tc[TOP_LOAD][3];
+#pragma unroll
+    for (int u = 0; u < TOP_LOAD; ++u) {
+      const uint32_t i = min(i0 + u * TOP_REFIT_THREADS + tid, T - 1);
+      L[u] = links[i];
+      const float4* t4 = reinterpret_cast<const float4*>(top_child + 12 * (size_t)i);
+      tc[u][0] = t4[0];
+      tc[u][1] = t4[1];
+      tc[u][2] = t4[2];
+    }
+#pragma unroll
+    for (int u = 0; u < TOP_LOAD; ++u) {
+      const uint32_t i = i0 + u * TOP_REFIT_THREADS + tid;
+      if (i >= T) continue;
+      const float a[6] = {tc[u][0].x, tc[u][0].y, tc[u][0].z, tc[u][0].w, tc[u][1].x, tc[u][1].y};
+      const float b[6] = {tc[u][1].z, tc[u][1].w, tc[u][2].x, tc[u][2].y, tc[u][2].z, tc[u][2].w};
+      float mn[3] = {100.0f, 100.0f, 100.0f}, mx[3] = {-100.0f, -100.0f, -100.0f};
+      if (L[u].x < 0) unite(mn, mx, a);
+      if (L[u].y < 0) unite(mn, mx, b);
+      put6(box + 6 * (size_t)i, mn, mx);
+      lnk[i] = make_int2(L[u].x, L[u].y);
+      lsl[i] = L[u].w;
+    }
   }
   __syncthreads();
   for (uint32_t h = 1; h <= H; ++h) {

@@ -310,7 +310,7 @@ struct Sim {
 
   // Direct (not captured) update of the body pose of frame g into its slot.
   void update_direct(uint64_t g, hipStream_t s) {
-    body.update(body.slot_of(g), 0, nullptr, g, s);
+    body.update(body.slot_of(g), 0, g, s);
   }
 
   // Moving body: drop the poses computed ahead and recompute the current
@@ -441,7 +441,7 @@ struct Sim {
         if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
         for (int j = l; j < G; j += body.nlanes) {
           const uint64_t f = frame + (uint64_t)(G + j);
-          body.update(body.slot_of(f), l, nullptr, f, ls);
+          body.update(body.slot_of(f), l, f, ls);
         }
         if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
         CC_HIP(hipEventRecord(ev_done[l], ls));
@@ -591,7 +591,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
                   2 * Sim::GRAPH_STEPS, lanes, s);
       // frame 0 into slot 0; a refitted body keeps the topology built from
       // the frame-0 pose in every slot
-      S.body.deform(0, nullptr, 0, s);
+      S.body.deform(0, 0, s);
       S.body.build(0, 0, s);
       S.body.share_topology(0, s);
       S.ready = 1;
@@ -978,7 +978,7 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     for (int it = 0; it < nsteps; ++it) {
       const int nk = S.body.slot_of(S.frame + 1);
       CC_HIP(hipEventRecord(ev[0], s));
-      if (S.body.moving()) S.body.deform(nk, nullptr, S.frame + 1, s);
+      if (S.body.moving()) S.body.deform(nk, S.frame + 1, s);
       CC_HIP(hipEventRecord(ev[1], s));
       if (S.body.mode == CC_BODY_REFIT) S.body.refit(nk, 0, s);
       else if (S.body.mode == CC_BODY_REBUILD) S.body.build(nk, 0, s);
