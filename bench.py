@@ -35,6 +35,39 @@ def algorithmic_bytes(sc):
     return {"verlet": verlet, "deform": deform, "refit": refit, "springs": E}
 
 
+def read_traffic(path):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a
+    plain copy of known size); None when the summary is absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes of freshly allocated (cold) memory,
+    through the HIP runtime's own copy kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    big = C.c_void_p()                    # evict a from the caches: touch 1 GiB of other memory
+    assert hip.hipMalloc(C.byref(big), C.c_size_t(1 << 30)) == 0
+    assert hip.hipMemset(big, 0, C.c_size_t(1 << 30)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpyDtoD(b, a, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    for ptr in (a, b, big):
+        hip.hipFree(ptr)
+
+
 def cpu_baseline(sc, budget_s=15.0, nthreads=1):
     """The oracle's FP32 path (the reference algorithm on the host) on a
     bounded sample of the same workload: whole steps (deform + refit + step)
@@ -89,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB of cold memory "
+                         "(the reference kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the cloth step is a "
@@ -162,8 +198,13 @@ def main():
         "bvh_nodes_visited_per_s": visits / el_c,
         "bvh_nodes_per_query": visits / max(queries, 1),
     }
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "k_verlet_large_traffic.txt"))
+    if traffic:
+        out["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, budget_s=args.cpu_budget)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib << 20)
     print(json.dumps(out))
 
 
