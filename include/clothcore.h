@@ -178,10 +178,12 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* body, const cc
 int cc_sim_destroy(cc_sim* sim);
 
 /* Simulator::simulate (simulation.cu:141-150) for nsteps frames: per frame
-   the body update (REFIT / REBUILD), then the fused Verlet + spring +
-   collision kernel (verlet.cu:129-161), then the ping-pong swap
-   (:243-251).  Asynchronous on stream (NULL: the simulator's own stream);
-   steps are replayed from a captured hipGraph. */
+   the fused Verlet + spring + collision kernel (verlet.cu:129-161) against
+   the body pose of that frame, then the ping-pong swap (:243-251).  A
+   REFIT / REBUILD body's poses and LBVHs are computed on the library's own
+   streams, up to 16 frames ahead of the cloth.  Asynchronous on stream
+   (NULL: the simulator's own stream); runs of 16 steps are replayed from a
+   captured hipGraph. */
 int cc_sim_step(cc_sim* sim, int nsteps, void* stream);
 int cc_sim_synchronize(cc_sim* sim);
 
@@ -223,8 +225,8 @@ int cc_sim_check_finite(cc_sim* sim);
 int cc_body_set_vertices(cc_sim* sim, const float* verts3);
 /* Current (deformed) body vertices, vec3. */
 int cc_body_get_vertices(cc_sim* sim, float* verts3);
-/* The float parameters the deformation used for the last body update:
-   R[9] row-major, T[3] (pivot folded in), breath[ncaps]. */
+/* The float parameters of the current frame's deformation: R[9]
+   row-major, T[3] (pivot folded in), breath[ncaps]. */
 int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath);
 /* BVHAccel construction (bvh.cu:264-297) / refit (calculateBoudingBox,
    cudaBRT.cu:166-206) over the current body vertices. */
