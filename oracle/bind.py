@@ -25,6 +25,12 @@ class or_params(C.Structure):
                 ("rest_scale", C.c_float), ("eps", C.c_float)]
 
 
+class or_params64(C.Structure):
+    _fields_ = [("dt", C.c_double), ("mass", C.c_double), ("gravity", C.c_double * 3),
+                ("damp", C.c_double), ("kd", C.c_double), ("k", C.c_double * 3),
+                ("rest_scale", C.c_double), ("eps", C.c_double)]
+
+
 class or_bvh(C.Structure):
     _fields_ = [("m", C.c_uint32), ("verts", C.c_void_p), ("leaf_tri", C.c_void_p),
                 ("childA", C.c_void_p), ("childB", C.c_void_p), ("box_int", C.c_void_p),
@@ -340,3 +346,69 @@ def body_deform(rest, vstart, p0, p1, breath, R, T):
                           _p(_c(breath, np.float32)), _p(_c(R, np.float32)), _p(_c(T, np.float32)),
                           _p(out))
     return out
+
+
+# ------------------------------------------------------------------ adjoint
+def params64_from(p, k=None, damp=None, rest_scale=None):
+    """Double-precision copy of cc_params-like p, with optional overrides of
+    the differentiable parameters."""
+    q = or_params64()
+    q.dt, q.mass, q.damp, q.kd = float(p.dt), float(p.mass), float(p.damp), float(p.kd)
+    for i in range(3):
+        q.gravity[i] = float(p.gravity[i])
+        q.k[i] = float(p.k[i])
+    q.rest_scale, q.eps = float(p.rest_scale), float(p.eps)
+    if k is not None:
+        for i in range(3):
+            q.k[i] = float(k[i])
+    if damp is not None:
+        q.damp = float(damp)
+    if rest_scale is not None:
+        q.rest_scale = float(rest_scale)
+    return q
+
+
+def rollout_f64(x0, v0, springs, prm64, pins=None, bvh=None, nsteps=1, trajectory=False):
+    """FP64 rollout; returns x_final, or (x_final, traj_p, traj_q, traj_c, traj_a)."""
+    n = x0.shape[0]
+    x0 = _c(x0, np.float64)
+    v0 = _c(np.zeros_like(x0) if v0 is None else v0, np.float64)
+    xf = np.zeros((n, 3), np.float64)
+    tp = tq = tc = to = None
+    if trajectory:
+        tp, tq, tc, to = (np.zeros((nsteps + 1, n, 3), np.float64) for _ in range(4))
+    sp = [(_c(r, np.uint32), _c(c, SPRING_DTYPE)) for r, c in springs]
+    pins_a = None if pins is None else _c(pins, np.uint8)
+    olib().or_rollout_f64(n, _p(x0), _p(v0), *_springs_args(sp), C.byref(prm64), _p(pins_a),
+                          bvh.ref if bvh is not None else None, nsteps, _p(xf), _p(tp), _p(tq), _p(tc), _p(to))
+    return (xf, tp, tq, tc, to) if trajectory else xf
+
+
+def rollout_f64_frozen(x0, v0, springs, prm64, pins, traj_c, traj_a):
+    """FP64 rollout with the recorded contacts held fixed; returns x_final."""
+    n = x0.shape[0]
+    nsteps = traj_c.shape[0] - 1
+    x0 = _c(x0, np.float64)
+    v0 = _c(np.zeros_like(x0) if v0 is None else v0, np.float64)
+    xf = np.zeros((n, 3), np.float64)
+    sp = [(_c(r, np.uint32), _c(c, SPRING_DTYPE)) for r, c in springs]
+    pins_a = None if pins is None else _c(pins, np.uint8)
+    olib().or_rollout_f64_frozen(n, _p(x0), _p(v0), *_springs_args(sp), C.byref(prm64), _p(pins_a), nsteps,
+                                 _p(_c(traj_c, np.float64)), _p(_c(traj_a, np.float64)), _p(xf))
+    return xf
+
+
+def adjoint_f64(springs, prm64, pins, traj_p, traj_q, traj_c, dl_dxf):
+    """FP64 adjoint along a recorded trajectory (contacts held fixed);
+    returns (dL/d(k0, k1, k2, damp, rest_scale), dL/dv0, dL/dx0)."""
+    nsteps = traj_p.shape[0] - 1
+    n = traj_p.shape[1]
+    sp = [(_c(r, np.uint32), _c(c, SPRING_DTYPE)) for r, c in springs]
+    pins_a = None if pins is None else _c(pins, np.uint8)
+    dp = np.zeros(5, np.float64)
+    dv0 = np.zeros((n, 3), np.float64)
+    dx0 = np.zeros((n, 3), np.float64)
+    olib().or_adjoint_f64(n, *_springs_args(sp), C.byref(prm64), _p(pins_a), nsteps,
+                          _p(_c(traj_p, np.float64)), _p(_c(traj_q, np.float64)), _p(_c(traj_c, np.float64)),
+                          _p(_c(dl_dxf, np.float64)), _p(dp), _p(dv0), _p(dx0))
+    return dp, dv0, dx0

@@ -687,8 +687,21 @@ void or_step_f32(uint32_t n, const float* pos, const float* prev, const float* c
                  pre, hit, inside);
 }
 
+void or_params_widen(const or_params* p, or_params64* q) {
+  q->dt = p->dt;
+  q->mass = p->mass;
+  for (int c = 0; c < 3; ++c) {
+    q->gravity[c] = p->gravity[c];
+    q->k[c] = p->k[c];
+  }
+  q->damp = p->damp;
+  q->kd = p->kd;
+  q->rest_scale = p->rest_scale;
+  q->eps = p->eps;
+}
+
 static v3d spring_row_f64(const double* pos, const double* prev, const uint32_t* row, const or_spring* col,
-                          uint32_t i, v3d p, v3d v, double ks, const or_params* P) {
+                          uint32_t i, v3d p, v3d v, double ks, const or_params64* P) {
   v3d f = {0.0, 0.0, 0.0};
   const double dt = P->dt, eps = P->eps, kd = P->kd, rs = P->rest_scale;
   for (uint32_t k = row[i]; k < row[i + 1]; ++k) {
@@ -711,13 +724,53 @@ void or_step_f64(uint32_t n, const double* pos, const double* prev, const double
                  const uint32_t* row0, const or_spring* col0,
                  const uint32_t* row1, const or_spring* col1,
                  const uint32_t* row2, const or_spring* col2,
-                 const or_params* P, const uint8_t* pins, const or_bvh* bvh,
+                 const or_params* P32, const uint8_t* pins, const or_bvh* bvh,
                  double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
                  int32_t* hit) {
+  or_params64 P;
+  or_params_widen(P32, &P);
+  or_step_f64p(n, pos, prev, contact, row0, col0, row1, col1, row2, col2, &P, pins, bvh, out_pos, out_prev,
+               out_contact, inside, hit);
+}
+
+/* verlet.cu:129-156 in FP64: the position before the collision response */
+static v3d pre_position_f64(uint32_t i, const double* pos, const double* prev, const double* contact,
+                            const uint32_t* const* rows, const or_spring* const* cols, const or_params64* P) {
+  const double dt = P->dt, mass = P->mass, damp = P->damp;
+  v3d p = d3(pos + 3 * (size_t)i), q = d3(prev + 3 * (size_t)i);
+  v3d vel = ddiv(dsub(p, q), dt);
+  v3d g = {P->gravity[0], P->gravity[1], P->gravity[2]};
+  v3d F = dadd(dmul(g, mass), dmul(vel, damp));
+  for (int t = 0; t < 3; ++t)
+    if (rows[t]) F = dadd(F, spring_row_f64(pos, prev, rows[t], cols[t], i, p, vel, P->k[t], P));
+  v3d c = d3(contact + 3 * (size_t)i);
+  F = dsub(F, dmul(c, ddot(c, F)));
+  v3d acc = ddiv(F, mass);
+  return dadd(dsub(dadd(p, p), q), dmul(dmul(acc, dt), dt));
+}
+
+void or_step_f64p(uint32_t n, const double* pos, const double* prev, const double* contact,
+                  const uint32_t* row0, const or_spring* col0,
+                  const uint32_t* row1, const or_spring* col1,
+                  const uint32_t* row2, const or_spring* col2,
+                  const or_params64* P, const uint8_t* pins, const or_bvh* bvh,
+                  double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
+                  int32_t* hit) {
+  or_step_f64_planes(n, pos, prev, contact, row0, col0, row1, col1, row2, col2, P, pins, bvh, out_pos, out_prev,
+                     out_contact, inside, hit, NULL);
+}
+
+void or_step_f64_planes(uint32_t n, const double* pos, const double* prev, const double* contact,
+                        const uint32_t* row0, const or_spring* col0,
+                        const uint32_t* row1, const or_spring* col1,
+                        const uint32_t* row2, const or_spring* col2,
+                        const or_params64* P, const uint8_t* pins, const or_bvh* bvh,
+                        double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
+                        int32_t* hit, double* plane_pt) {
   const uint32_t* rows[3] = {row0, row1, row2};
   const or_spring* cols[3] = {col0, col1, col2};
-  const double dt = P->dt, mass = P->mass, damp = P->damp;
   for (uint32_t i = 0; i < n; ++i) {
+    if (plane_pt) plane_pt[3 * (size_t)i] = plane_pt[3 * (size_t)i + 1] = plane_pt[3 * (size_t)i + 2] = 0.0;
     v3d p = d3(pos + 3 * (size_t)i), q = d3(prev + 3 * (size_t)i);
     if (pins && pins[i]) {
       for (int c = 0; c < 3; ++c) {
@@ -729,15 +782,8 @@ void or_step_f64(uint32_t n, const double* pos, const double* prev, const double
       if (hit) hit[i] = -1;
       continue;
     }
-    v3d vel = ddiv(dsub(p, q), dt);
-    v3d g = {P->gravity[0], P->gravity[1], P->gravity[2]};
-    v3d F = dadd(dmul(g, mass), dmul(vel, damp));
-    for (int t = 0; t < 3; ++t)
-      if (rows[t]) F = dadd(F, spring_row_f64(pos, prev, rows[t], cols[t], i, p, vel, (double)P->k[t], P));
-    v3d c = d3(contact + 3 * (size_t)i);
-    F = dsub(F, dmul(c, ddot(c, F)));
-    v3d acc = ddiv(F, mass);
-    v3d np = dadd(dsub(dadd(p, p), q), dmul(dmul(acc, dt), dt));
+    (void)q;
+    v3d np = pre_position_f64(i, pos, prev, contact, rows, cols, P);
     v3d nq = p;
     v3d nc = {0.0, 0.0, 0.0};
     uint8_t ins = 0;
@@ -758,6 +804,11 @@ void or_step_f64(uint32_t n, const double* pos, const double* prev, const double
           nq = np;
           nc = nn;
           ins = 1;
+          if (plane_pt) {
+            plane_pt[3 * (size_t)i] = a.x;
+            plane_pt[3 * (size_t)i + 1] = a.y;
+            plane_pt[3 * (size_t)i + 2] = a.z;
+          }
         }
       }
     }
@@ -767,6 +818,185 @@ void or_step_f64(uint32_t n, const double* pos, const double* prev, const double
     out_prev[3 * (size_t)i] = nq.x; out_prev[3 * (size_t)i + 1] = nq.y; out_prev[3 * (size_t)i + 2] = nq.z;
     out_contact[3 * (size_t)i] = nc.x; out_contact[3 * (size_t)i + 1] = nc.y; out_contact[3 * (size_t)i + 2] = nc.z;
   }
+}
+
+void or_rollout_f64(uint32_t n, const double* x0, const double* v0,
+                    const uint32_t* row0, const or_spring* col0,
+                    const uint32_t* row1, const or_spring* col1,
+                    const uint32_t* row2, const or_spring* col2,
+                    const or_params64* P, const uint8_t* pins, const or_bvh* bvh, uint32_t nsteps,
+                    double* x_final, double* traj_p, double* traj_q, double* traj_c, double* traj_a) {
+  const size_t N = 3 * (size_t)n;
+  double* buf = (double*)calloc(6 * N, sizeof(double));
+  double *p = buf, *q = buf + N, *c = buf + 2 * N, *np = buf + 3 * N, *nq = buf + 4 * N, *nc = buf + 5 * N;
+  for (size_t k = 0; k < N; ++k) {
+    p[k] = x0[k];
+    q[k] = x0[k] - (v0 ? v0[k] : 0.0) * P->dt;
+  }
+  for (uint32_t t = 0; t <= nsteps; ++t) {
+    if (traj_p) memcpy(traj_p + t * N, p, N * sizeof(double));
+    if (traj_q) memcpy(traj_q + t * N, q, N * sizeof(double));
+    if (traj_c) memcpy(traj_c + t * N, c, N * sizeof(double));
+    if (t == nsteps) break;
+    or_step_f64_planes(n, p, q, c, row0, col0, row1, col1, row2, col2, P, pins, bvh, np, nq, nc, NULL, NULL,
+                       traj_a ? traj_a + (t + 1) * N : NULL);
+    double* tmp;
+    tmp = p; p = np; np = tmp;
+    tmp = q; q = nq; nq = tmp;
+    tmp = c; c = nc; nc = tmp;
+  }
+  memcpy(x_final, p, N * sizeof(double));
+  free(buf);
+}
+
+void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
+                           const uint32_t* row0, const or_spring* col0,
+                           const uint32_t* row1, const or_spring* col1,
+                           const uint32_t* row2, const or_spring* col2,
+                           const or_params64* P, const uint8_t* pins, uint32_t nsteps,
+                           const double* traj_c, const double* traj_a, double* x_final) {
+  const uint32_t* rows[3] = {row0, row1, row2};
+  const or_spring* cols[3] = {col0, col1, col2};
+  const size_t N = 3 * (size_t)n;
+  double* buf = (double*)calloc(4 * N, sizeof(double));
+  double *p = buf, *q = buf + N, *np = buf + 2 * N, *nq = buf + 3 * N;
+  for (size_t k = 0; k < N; ++k) {
+    p[k] = x0[k];
+    q[k] = x0[k] - (v0 ? v0[k] : 0.0) * P->dt;
+  }
+  for (uint32_t t = 0; t < nsteps; ++t) {
+    const double* c = traj_c + t * N;
+    const double* cn = traj_c + (t + 1) * N;
+    const double* an = traj_a + (t + 1) * N;
+    for (uint32_t i = 0; i < n; ++i) {
+      v3d xp = d3(p + 3 * (size_t)i), u, uq;
+      if (pins && pins[i]) {
+        u = xp;
+        uq = xp;
+      } else {
+        u = pre_position_f64(i, p, q, c, rows, cols, P);
+        uq = xp;
+        const v3d nn = d3(cn + 3 * (size_t)i);
+        if (nn.x != 0.0 || nn.y != 0.0 || nn.z != 0.0) {   /* the recorded plane */
+          const double d = ddot(dsub(u, d3(an + 3 * (size_t)i)), nn);
+          u = dadd(u, dmul(nn, -d));     /* = |d| n on the side the forward saw */
+          uq = u;
+        }
+      }
+      np[3 * (size_t)i] = u.x; np[3 * (size_t)i + 1] = u.y; np[3 * (size_t)i + 2] = u.z;
+      nq[3 * (size_t)i] = uq.x; nq[3 * (size_t)i + 1] = uq.y; nq[3 * (size_t)i + 2] = uq.z;
+    }
+    double* tmp;
+    tmp = p; p = np; np = tmp;
+    tmp = q; q = nq; nq = tmp;
+  }
+  memcpy(x_final, p, N * sizeof(double));
+  free(buf);
+}
+
+/* Vector-Jacobian product of one spring force f = alpha * d/L,
+   alpha = -ks (L - rest) + kd (dv . d)/L, d = p_i - p_j + eps, with the
+   weight lam: g_d = (df/dd)^T lam, g_dv = (df/ddv)^T lam; dl = (d/L) . lam. */
+static void spring_vjp_f64(v3d d, v3d dv, double ks, double rest, double kd, v3d lam, v3d* gd, v3d* gdv,
+                           double* dl, double* len) {
+  const double L = sqrt(ddot(d, d));
+  const v3d u = dmul(d, 1.0 / L);
+  const double ud = ddot(u, dv);
+  const double alpha = -ks * (L - rest) + kd * ud;
+  const double l = ddot(u, lam);
+  /* d alpha / d d = -ks u + kd (dv - (u.dv) u) / L;  d u / d d = (I - u u^T) / L */
+  const v3d da = dadd(dmul(u, -ks), dmul(dsub(dv, dmul(u, ud)), kd / L));
+  *gd = dadd(dmul(da, l), dmul(dsub(lam, dmul(u, l)), alpha / L));
+  *gdv = dmul(u, kd * l);
+  *dl = l;
+  *len = L;
+}
+
+void or_adjoint_f64(uint32_t n,
+                    const uint32_t* row0, const or_spring* col0,
+                    const uint32_t* row1, const or_spring* col1,
+                    const uint32_t* row2, const or_spring* col2,
+                    const or_params64* P, const uint8_t* pins, uint32_t nsteps,
+                    const double* traj_p, const double* traj_q, const double* traj_c,
+                    const double* dl_dxf, double dparams[5], double* dl_dv0, double* dl_dx0) {
+  const uint32_t* rows[3] = {row0, row1, row2};
+  const or_spring* cols[3] = {col0, col1, col2};
+  const size_t N = 3 * (size_t)n;
+  const double dt = P->dt, w = dt * dt / P->mass;
+  double* buf = (double*)calloc(5 * N, sizeof(double));
+  double *lp = buf, *lq = buf + N, *np = buf + 2 * N, *nq = buf + 3 * N, *lf = buf + 4 * N;
+  double g[5] = {0, 0, 0, 0, 0};
+  memcpy(lp, dl_dxf, N * sizeof(double));
+  for (uint32_t tt = nsteps; tt-- > 0;) {
+    const double* p = traj_p + tt * N;
+    const double* q = traj_q + tt * N;
+    const double* c = traj_c + tt * N;
+    const double* cn = traj_c + (tt + 1) * N;
+    memset(np, 0, 2 * N * sizeof(double));
+    memset(lf, 0, N * sizeof(double));
+    for (uint32_t i = 0; i < n; ++i) {
+      const v3d lP = d3(lp + 3 * (size_t)i), lQ = d3(lq + 3 * (size_t)i);
+      v3d gp = {0, 0, 0}, gq = {0, 0, 0};
+      if (pins && pins[i]) {          /* x' = x'_prev = x */
+        gp = dadd(lP, lQ);
+      } else {
+        const v3d nn = d3(cn + 3 * (size_t)i);
+        v3d lu;
+        if (nn.x != 0.0 || nn.y != 0.0 || nn.z != 0.0) {   /* projected: x' = x'_prev = plane projection of u */
+          const v3d s2 = dadd(lP, lQ);
+          lu = dsub(s2, dmul(nn, ddot(nn, s2)));
+        } else {                                         /* x' = u, x'_prev = x */
+          lu = lP;
+          gp = lQ;
+        }
+        gp = dadd(gp, dmul(lu, 2.0));                    /* u = 2x - x_prev + w (I - c c^T) F */
+        gq = dmul(lu, -1.0);
+        const v3d cc = d3(c + 3 * (size_t)i);
+        const v3d lF = dmul(dsub(lu, dmul(cc, ddot(cc, lu))), w);
+        lf[3 * (size_t)i] = lF.x; lf[3 * (size_t)i + 1] = lF.y; lf[3 * (size_t)i + 2] = lF.z;
+        const v3d vel = ddiv(dsub(d3(p + 3 * (size_t)i), d3(q + 3 * (size_t)i)), dt);
+        g[3] += ddot(lF, vel);                           /* F += damp * vel */
+        gp = dadd(gp, dmul(lF, P->damp / dt));
+        gq = dsub(gq, dmul(lF, P->damp / dt));
+      }
+      np[3 * (size_t)i] += gp.x; np[3 * (size_t)i + 1] += gp.y; np[3 * (size_t)i + 2] += gp.z;
+      nq[3 * (size_t)i] += gq.x; nq[3 * (size_t)i + 1] += gq.y; nq[3 * (size_t)i + 2] += gq.z;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      if (pins && pins[i]) continue;
+      const v3d lF = d3(lf + 3 * (size_t)i);
+      const v3d pi = d3(p + 3 * (size_t)i), vi = ddiv(dsub(pi, d3(q + 3 * (size_t)i)), dt);
+      for (int t = 0; t < 3; ++t) {
+        if (!rows[t]) continue;
+        for (uint32_t k = rows[t][i]; k < rows[t][i + 1]; ++k) {
+          const uint32_t j = (uint32_t)cols[t][k].end;
+          const v3d pj = d3(p + 3 * (size_t)j), vj = ddiv(dsub(pj, d3(q + 3 * (size_t)j)), dt);
+          v3d d = dsub(pi, pj);
+          d.x += P->eps; d.y += P->eps; d.z += P->eps;
+          const double rest0 = (double)cols[t][k].original, rest = rest0 * P->rest_scale;
+          v3d gd, gdv;
+          double l, L;
+          spring_vjp_f64(d, dsub(vi, vj), P->k[t], rest, P->kd, lF, &gd, &gdv, &l, &L);
+          const v3d a = dadd(gd, ddiv(gdv, dt)), b = ddiv(gdv, dt);
+          np[3 * (size_t)i] += a.x; np[3 * (size_t)i + 1] += a.y; np[3 * (size_t)i + 2] += a.z;
+          nq[3 * (size_t)i] -= b.x; nq[3 * (size_t)i + 1] -= b.y; nq[3 * (size_t)i + 2] -= b.z;
+          np[3 * (size_t)j] -= a.x; np[3 * (size_t)j + 1] -= a.y; np[3 * (size_t)j + 2] -= a.z;
+          nq[3 * (size_t)j] += b.x; nq[3 * (size_t)j + 1] += b.y; nq[3 * (size_t)j + 2] += b.z;
+          g[t] += -(L - rest) * l;
+          g[4] += P->k[t] * rest0 * l;
+        }
+      }
+    }
+    double* tmp;
+    tmp = lp; lp = np; np = tmp;
+    tmp = lq; lq = nq; nq = tmp;
+  }
+  if (dparams) memcpy(dparams, g, sizeof(g));
+  for (size_t k = 0; k < N; ++k) {   /* x_cur = x0, x_prev = x0 - v0 dt */
+    if (dl_dx0) dl_dx0[k] = lp[k] + lq[k];
+    if (dl_dv0) dl_dv0[k] = -dt * lq[k];
+  }
+  free(buf);
 }
 
 void or_collide_points(uint32_t n, const float* pts, const or_bvh* bvh,

@@ -56,6 +56,18 @@ typedef struct {
   float eps;         /* bias added to deltaP (verlet.cu:73, FLT_EPSILON) */
 } or_params;
 
+/* The same parameters in double, for the FP64 path (finite differences
+   need perturbations below float resolution). */
+typedef struct {
+  double dt, mass;
+  double gravity[3];
+  double damp, kd;
+  double k[3];
+  double rest_scale, eps;
+} or_params64;
+
+void or_params_widen(const or_params* p, or_params64* out);
+
 /* A read-only view of a built LBVH, reference encoding. */
 typedef struct {
   uint32_t m;               /* leaves */
@@ -144,6 +156,66 @@ void or_step_f64(uint32_t n, const double* pos, const double* prev, const double
                  const or_params* prm, const uint8_t* pins, const or_bvh* bvh,
                  double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
                  int32_t* hit);
+
+/* or_step_f64 with double parameters (or_step_f64 widens and calls it). */
+void or_step_f64p(uint32_t n, const double* pos, const double* prev, const double* contact,
+                  const uint32_t* row0, const or_spring* col0,
+                  const uint32_t* row1, const or_spring* col1,
+                  const uint32_t* row2, const or_spring* col2,
+                  const or_params64* prm, const uint8_t* pins, const or_bvh* bvh,
+                  double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
+                  int32_t* hit);
+
+/* or_step_f64p that also returns, for every projected vertex, the point a
+   of its plane (the tested triangle's first vertex; 0 for the others;
+   plane_pt may be NULL). */
+void or_step_f64_planes(uint32_t n, const double* pos, const double* prev, const double* contact,
+                        const uint32_t* row0, const or_spring* col0,
+                        const uint32_t* row1, const or_spring* col1,
+                        const uint32_t* row2, const or_spring* col2,
+                        const or_params64* prm, const uint8_t* pins, const or_bvh* bvh,
+                        double* out_pos, double* out_prev, double* out_contact, uint8_t* inside,
+                        int32_t* hit, double* plane_pt);
+
+/* FP64 rollout of nsteps frames from x_cur = x0, x_prev = x0 - v0*dt,
+   contact 0.  traj_p / traj_q / traj_c (optional, (nsteps+1) x 3n each)
+   receive the state entering every step and the final one (traj_c[t]: the
+   contact normals entering step t; traj_c[t+1]: those it produced);
+   traj_a (optional, (nsteps+1) x 3n) the plane points of step t's
+   projections at row t+1. */
+void or_rollout_f64(uint32_t n, const double* x0, const double* v0,
+                    const uint32_t* row0, const or_spring* col0,
+                    const uint32_t* row1, const or_spring* col1,
+                    const uint32_t* row2, const or_spring* col2,
+                    const or_params64* prm, const uint8_t* pins, const or_bvh* bvh, uint32_t nsteps,
+                    double* x_final, double* traj_p, double* traj_q, double* traj_c, double* traj_a);
+
+/* The rollout with the contacts of a recorded one held fixed: a vertex with
+   a contact normal n at traj_c[t+1] is projected onto the plane through
+   traj_a[t+1] with normal n in step t whatever its position, the others
+   never are; the force projection uses traj_c[t].  The function whose
+   derivative or_adjoint_f64 computes; at the recorded parameters it equals
+   the rollout bit for bit. */
+void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
+                           const uint32_t* row0, const or_spring* col0,
+                           const uint32_t* row1, const or_spring* col1,
+                           const uint32_t* row2, const or_spring* col2,
+                           const or_params64* prm, const uint8_t* pins, uint32_t nsteps,
+                           const double* traj_c, const double* traj_a, double* x_final);
+
+/* Adjoint of or_rollout_f64 along a recorded trajectory, with the contacts
+   of the forward held fixed: a vertex projected in a step is moved by the
+   fixed plane projection (I - n n^T) u + (n . a) n, and the contact normal
+   that enters the next step's force projection is a constant.  Given
+   dL/dx_final (3n), returns dL/d(k0, k1, k2, damp, rest_scale), dL/dv0 and
+   dL/dx0 (3n each; any output may be NULL). */
+void or_adjoint_f64(uint32_t n,
+                    const uint32_t* row0, const or_spring* col0,
+                    const uint32_t* row1, const or_spring* col1,
+                    const uint32_t* row2, const or_spring* col2,
+                    const or_params64* prm, const uint8_t* pins, uint32_t nsteps,
+                    const double* traj_p, const double* traj_q, const double* traj_c,
+                    const double* dl_dxf, double dparams[5], double* dl_dv0, double* dl_dx0);
 
 /* collision query + projection for a batch of points (no Verlet) */
 void or_collide_points(uint32_t n, const float* pts, const or_bvh* bvh,

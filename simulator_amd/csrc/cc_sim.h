@@ -1,0 +1,269 @@
+// cc_sim.h -- the simulator object behind cc_sim: cloth buffers, the
+// collider body, the step graphs and the update lanes.
+#pragma once
+
+#include <vector>
+
+#include "cc_body.h"
+
+namespace cc {
+
+constexpr int SPRING_BATCH = 8;
+constexpr uint32_t PIN_BIT = 0x80000000u;
+constexpr uint32_t ROW_MASK = 0x7fffffffu;
+constexpr uint32_t END_MASK = 0x3fffffffu;
+
+struct StepArgs {
+  float inv_dt, dt2, inv_mass;
+  float gmx, gmy, gmz;  // gravity * mass
+  float damp, kd, k0, k1, k2, rest_scale, eps;
+  uint32_t ntypes;
+};
+
+StepArgs step_args(const cc_params& prm, uint32_t ntypes);
+
+// ------------------------------------------------------------------ Sim
+struct Sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, ntypes = 0;
+  cc_params prm{};
+  Body body;
+  DBuf<float> state[2];
+  DBuf<float> contact;
+  DBuf<uint32_t> row;
+  DBuf<int32_t> spill;      // [STACK_SPILL * n] traversal stack spill of the step kernel
+  DBuf<uint2> rec;
+  std::vector<uint32_t> h_row;
+  int cur = 0;              // state[cur] holds frame `frame`
+  uint64_t frame = 0;
+  uint64_t ready = 0;       // moving body: the poses of frames [frame, ready) are in their slots
+  bool debug = false, counting = false;
+  DBuf<float> dbg_pre;
+  DBuf<int32_t> dbg_hit;
+  DBuf<uint8_t> dbg_in;
+  DBuf<unsigned long long> ctr;
+  DBuf<uint32_t> flag;
+  // VBO
+  uint32_t nf = 0;
+  DBuf<uint32_t> faces, vrow, vcol;
+  DBuf<float> face_n, vnormals;
+
+  // Graphs of GRAPH_STEPS steps, per slot of the first frame and state
+  // parity (captured when first needed).
+  static constexpr int GRAPH_STEPS = 16;
+  hipGraphExec_t gmain[MAX_SLOTS][2] = {};
+  // update lanes (moving bodies): streams and the events that order them
+  // against the step stream
+  hipStream_t lane_s[MAX_LANES] = {};
+  hipEvent_t ev_go = nullptr, ev_done[MAX_LANES] = {};
+  bool lanes_inflight = false;
+  // Timing (cc_sim_set_timing): event set e brackets the step graph
+  // (tev[e][0..1]) and each lane's updates (tev[e][2+2l..3+2l]) on the
+  // stream they run on.  Two sets alternate, so the host reads one block's events while
+  // the next block runs.
+  bool timing = false;
+  hipEvent_t tev[2][2 + 2 * MAX_LANES] = {};
+  bool tpending[2] = {false, false};
+  int tset = 0;
+  double t_verlet_ms = 0, t_update_ms = 0;
+  uint64_t t_steps = 0, t_updates = 0;
+
+  ~Sim() {
+    drop_graphs();
+    for (auto& set : tev)
+      for (hipEvent_t e : set)
+        if (e) (void)hipEventDestroy(e);
+    if (ev_go) (void)hipEventDestroy(ev_go);
+    for (int l = 0; l < MAX_LANES; ++l) {
+      if (ev_done[l]) (void)hipEventDestroy(ev_done[l]);
+      if (lane_s[l]) (void)hipStreamDestroy(lane_s[l]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void drop_graphs() {
+    for (auto& p : gmain)
+      for (hipGraphExec_t& g : p) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
+
+  StepArgs args() const { return step_args(prm, ntypes); }
+
+  // the fused step reading body slot k and state[cur], writing state[cur ^ 1]
+  void launch_verlet(int k, hipStream_t s);
+  // the plain fused step between caller buffers (no debug outputs, no
+  // counters), with the given arguments
+  void launch_step(const float* in, float* out, float* contact, const StepArgs& a, int k, hipStream_t s);
+
+  // Direct (not captured) update of the body pose of frame g into its slot.
+  void update_direct(uint64_t g, hipStream_t s) {
+    body.update(body.slot_of(g), 0, g, s);
+  }
+
+  // Moving body: drop the poses computed ahead and recompute the current
+  // frame's (after a change of frame, parameters or vertices).  Synchronous.
+  void refresh_body() {
+    if (body.moving()) {
+      update_direct(frame, stream);
+      ready = frame + 1;
+    } else if (body.mode == CC_BODY_STATIC) {
+      body.refit(0, 0, stream);
+    }
+    CC_HIP(hipStreamSynchronize(stream));
+  }
+
+  hipGraphExec_t end_capture(int cur0) {
+    cur = cur0;
+    hipGraph_t graph = nullptr;
+    CC_HIP(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t g = nullptr;
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw Error(CC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return g;
+  }
+
+  template <class F>
+  hipGraphExec_t capture(F&& body_of_graph) {
+    const int cur0 = cur;
+    CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      body_of_graph();
+    } catch (...) {
+      cur = cur0;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    return end_capture(cur0);
+  }
+
+  // GRAPH_STEPS fused steps from a frame in slot k0, starting at the current
+  // state parity.
+  hipGraphExec_t capture_main(int k0) {
+    return capture([&] {
+      for (int j = 0; j < GRAPH_STEPS; ++j) {
+        launch_verlet(body.slot_of((uint64_t)(k0 + j)), stream);
+        cur ^= 1;
+      }
+    });
+  }
+
+  // read the events of a finished timed block
+  void collect(int set) {
+    if (!tpending[set]) return;
+    hipEvent_t* te = tev[set];
+    const int G = GRAPH_STEPS;
+    CC_HIP(hipEventSynchronize(te[1]));
+    float ms = 0;
+    CC_HIP(hipEventElapsedTime(&ms, te[0], te[1]));
+    t_verlet_ms += ms;
+    t_steps += (uint64_t)G;
+    if (body.moving())
+      for (int l = 0; l < body.nlanes; ++l) {
+        CC_HIP(hipEventSynchronize(te[3 + 2 * l]));
+        CC_HIP(hipEventElapsedTime(&ms, te[2 + 2 * l], te[3 + 2 * l]));
+        t_update_ms += ms;
+        t_updates += (uint64_t)((G - l + body.nlanes - 1) / body.nlanes);
+      }
+    tpending[set] = false;
+  }
+
+  // the step stream waits for the update lanes of the last block
+  void join_lanes(hipStream_t s) {
+    if (!lanes_inflight) return;
+    for (int l = 0; l < body.nlanes; ++l) CC_HIP(hipStreamWaitEvent(s, ev_done[l], 0));
+    lanes_inflight = false;
+  }
+
+  void step_single(hipStream_t s) {
+    join_lanes(s);
+    launch_verlet(body.slot_of(frame), s);
+    cur ^= 1;
+    ++frame;
+    if (body.moving() && ready <= frame) {
+      update_direct(frame, s);
+      ready = frame + 1;
+    }
+  }
+
+  // One block of GRAPH_STEPS steps.  A moving body's poses for these frames
+  // are in their slots (the previous block's lanes computed them, or they
+  // are computed here first); the lanes compute the poses of the next
+  // GRAPH_STEPS frames, into the slots of the last GRAPH_STEPS frames (the
+  // ring holds 2 * GRAPH_STEPS), while the steps run:
+  //
+  //   step stream:  join(lanes of last block) -> go -> [step graph: 16 fused steps]
+  //   lane l:       wait(go) -> U(f+16+l), U(f+16+l+L), ... -> done[l]
+  //
+  // A dependency between streams costs several microseconds, so a block has
+  // two, not two per step.  "go" follows the previous step graph, so a lane
+  // overwrites a slot only after the steps that read it.
+  void step_graph(hipStream_t s) {
+    const int G = GRAPH_STEPS;
+    const int k0 = body.slot_of(frame);
+    hipEvent_t* te = timing ? tev[tset] : nullptr;
+    if (timing) collect(tset);
+    const bool pipe = body.moving();
+    if (pipe) {
+      join_lanes(s);
+      for (; ready < frame + G; ++ready) update_direct(ready, s);
+      CC_HIP(hipEventRecord(ev_go, s));
+    }
+    // The step graph is submitted first: submitting a graph to a stream that
+    // waits on another stream can hold the host until that wait is over,
+    // and the lanes' wait ("go") is over as soon as the step graph starts.
+    hipGraphExec_t& g = gmain[k0][cur];
+    if (!g) g = capture_main(k0);
+    if (te) CC_HIP(hipEventRecord(te[0], s));
+    CC_HIP(hipGraphLaunch(g, s));
+    if (te) CC_HIP(hipEventRecord(te[1], s));
+    if (pipe) {
+      // the lanes are launched directly: a graph launch can hold the host
+      // for most of the graph's run, which delays the next lane
+      for (int l = 0; l < body.nlanes; ++l) {
+        hipStream_t ls = lane_s[l];
+        CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
+        if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
+        for (int j = l; j < G; j += body.nlanes) {
+          const uint64_t f = frame + (uint64_t)(G + j);
+          body.update(body.slot_of(f), l, f, ls);
+        }
+        if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
+        CC_HIP(hipEventRecord(ev_done[l], ls));
+      }
+      lanes_inflight = true;
+      ready = frame + 2 * G;
+    }
+    if (te) {
+      tpending[tset] = true;
+      tset ^= 1;
+    }
+    frame += (uint64_t)G;   // G is even: the state parity is unchanged
+  }
+
+  // Runs of GRAPH_STEPS steps go through the graphs; the rest are launched
+  // one by one, each followed by the body update of the next frame when it
+  // is not ready yet.
+  void step(int nsteps, hipStream_t s) {
+    if (!s) s = stream;
+    while (nsteps > 0) {
+      if (nsteps >= GRAPH_STEPS) {
+        step_graph(s);
+        nsteps -= GRAPH_STEPS;
+      } else {
+        step_single(s);
+        --nsteps;
+      }
+    }
+  }
+};
+
+}  // namespace cc
+
+struct cc_sim {
+  cc::Sim s;
+};
