@@ -292,6 +292,30 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
 int cc_sim_set_timing(cc_sim* sim, int enable);
 int cc_sim_get_timing(cc_sim* sim, cc_timing* out);
 
+/* -------------------------------------------------------------- adjoint */
+
+/* Reverse-mode gradient of a rollout (no counterpart in the reference: the
+   differentiable pattern beside the plain step).  The rollout runs nsteps
+   frames of the fused step over sim's cloth, springs, pins and body (static
+   or none) from x_cur = x0, x_prev = x0 - v0*dt, contact 0, with
+   theta = (k0, k1, k2, damp, rest_scale) replacing those of sim's
+   parameters.  The gradient holds the forward's contacts fixed: a vertex
+   projected in a step is moved by that fixed plane projection, and the
+   contact normal entering the next step's force projection is a constant.
+   The spring set must be symmetric (each i->j with a j->i of the same type
+   and rest length).  States are checkpointed every ckpt_every steps and
+   recomputed in the backward pass.  sim must outlive the handle. */
+typedef struct cc_adj cc_adj;
+int cc_adj_create(cc_sim* sim, uint32_t nsteps, uint32_t ckpt_every, cc_adj** out);
+int cc_adj_destroy(cc_adj* adj);
+/* Forward rollout; x_final (3n floats, may be NULL) receives the final
+   positions; v0 may be NULL (zero). */
+int cc_adj_forward(cc_adj* adj, const float* x0, const float* v0, const double theta[5], float* x_final);
+/* Backward of the last forward: from dL/dx_final (3n) to dL/dtheta (FP64,
+   reduced in FP64 on the GPU), dL/dv0 and dL/dx0 (3n each; outputs may be
+   NULL). */
+int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], float* dl_dv0, float* dl_dx0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,10 @@ PROTOTYPES = {
     "cc_sim_get_normals": (C.c_int, [P, P]),
     "cc_sim_time_kernels": (C.c_int, [P, C.c_int, C.POINTER(cc_timing)]),
     "cc_sim_set_timing": (C.c_int, [P, C.c_int]),
+    "cc_adj_create": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(P)]),
+    "cc_adj_destroy": (C.c_int, [P]),
+    "cc_adj_forward": (C.c_int, [P, P, P, P, P]),
+    "cc_adj_backward": (C.c_int, [P, P, P, P, P]),
     "cc_sim_get_timing": (C.c_int, [P, C.POINTER(cc_timing)]),
 }
 
