@@ -316,6 +316,31 @@ int cc_adj_forward(cc_adj* adj, const float* x0, const float* v0, const double t
    NULL). */
 int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], float* dl_dv0, float* dl_dx0);
 
+/* -------------------------------------------------------------- batched */
+
+/* Many independent instances, each a cloth with its own springs, parameters
+   and body, advanced together: one launch per step covers every instance,
+   runs of 16 steps replay a captured hipGraph.  The reference runs one
+   Simulator per scene (simulator.h:9-59); a batch equals count separate
+   cc_sim handles stepped in turn, bit for bit.  The workgroup size and the
+   order of the workgroups are picked at creation by timing each candidate
+   (cc_batch_info reports the choice).  bodies may be NULL (no colliders) or
+   hold count entries of mode NONE or STATIC. */
+typedef struct cc_batch cc_batch;
+int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, const cc_params* params, uint32_t count,
+                    cc_batch** out);
+int cc_batch_destroy(cc_batch* b);
+/* cc_sim_step for every instance (asynchronous on stream, NULL: the batch's
+   own stream). */
+int cc_batch_step(cc_batch* b, int nsteps, void* stream);
+int cc_batch_synchronize(cc_batch* b);
+/* cc_sim_set_pins / cc_sim_get_state for instance inst. */
+int cc_batch_set_pins(cc_batch* b, uint32_t inst, const uint32_t* idx, uint32_t k);
+int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, float* contact_n);
+/* Total vertices, the chosen workgroup size (64, 128 or 256) and order
+   (0: instance order, 1: largest instance first); any pointer may be NULL. */
+int cc_batch_info(cc_batch* b, uint32_t* vertices, uint32_t* tile, uint32_t* order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,6 +126,13 @@ PROTOTYPES = {
     "cc_adj_destroy": (C.c_int, [P]),
     "cc_adj_forward": (C.c_int, [P, P, P, P, P]),
     "cc_adj_backward": (C.c_int, [P, P, P, P, P]),
+    "cc_batch_create": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
+    "cc_batch_destroy": (C.c_int, [P]),
+    "cc_batch_step": (C.c_int, [P, C.c_int, P]),
+    "cc_batch_synchronize": (C.c_int, [P]),
+    "cc_batch_set_pins": (C.c_int, [P, C.c_uint32, P, C.c_uint32]),
+    "cc_batch_get_state": (C.c_int, [P, C.c_uint32, P, P, P]),
+    "cc_batch_info": (C.c_int, [P, P, P, P]),
     "cc_sim_get_timing": (C.c_int, [P, C.POINTER(cc_timing)]),
 }
 

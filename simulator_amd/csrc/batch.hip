@@ -1,0 +1,392 @@
+// batch.hip -- many independent cloth / body instances in one launch per step.
+//
+// Every instance's cloth, springs and body LBVH are concatenated in flat
+// device arrays.  A workgroup covers a range of one instance's vertices, so
+// the instance's step arguments and LBVH view (a per-instance descriptor)
+// are uniform across the workgroup.  One launch advances all instances by
+// one step with the same per-vertex code as the single-cloth kernel
+// (step_vertex), and 16 steps are replayed from a captured graph.  The
+// workgroup size (64, 128 or 256 vertices: one to four wavefronts per row
+// segment) and the order of the workgroups (instance order, or largest
+// instance first) are chosen at creation by timing each candidate on a copy
+// of the initial state.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "cc_step.h"
+
+namespace cc {
+
+struct InstDesc {
+  StepArgs a;
+  BvhView bvh;
+};
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__ sin_, float* __restrict__ sout,
+                                                       float* __restrict__ contact, const uint32_t* __restrict__ row,
+                                                       const uint2* __restrict__ rec,
+                                                       const InstDesc* __restrict__ insts,
+                                                       const uint4* __restrict__ blocks, uint32_t ntotal,
+                                                       int32_t* __restrict__ spill) {
+  const uint4 b = blocks[blockIdx.x];     // {instance, first vertex, vertex count, -}
+  if (threadIdx.x >= b.z) return;
+  const InstDesc d = insts[b.x];
+  const uint32_t i = b.y + threadIdx.x;
+  uint32_t visits = 0;
+  step_vertex<false>(i, sin_, sout, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
+                     nullptr, nullptr);
+}
+
+constexpr int TILES[3] = {64, 128, 256};
+constexpr int GRAPH_STEPS_B = 16;
+
+struct Batch {
+  hipStream_t stream = nullptr;
+  uint32_t count = 0, ntotal = 0;
+  std::vector<uint32_t> v_off;               // [count+1] first vertex of each instance
+  std::vector<uint32_t> h_row;
+  DBuf<float> state[2], contact;
+  DBuf<uint32_t> row;
+  DBuf<uint2> rec;
+  DBuf<int32_t> spill;
+  // bodies, concatenated
+  DBuf<Node> nodes;
+  DBuf<uint4> leaf_tri;
+  DBuf<float> verts, leaf0;
+  DBuf<InstDesc> insts;
+  // the chosen launch shape
+  int tile = 256, order = 0;
+  DBuf<uint4> blocks;
+  uint32_t nblocks = 0;
+  int cur = 0;
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+
+  ~Batch() {
+    for (hipGraphExec_t& g : graphs)
+      if (g) (void)hipGraphExecDestroy(g);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // the workgroups for a tile size and an order (0: instance order,
+  // 1: instances by decreasing vertex count)
+  std::vector<uint4> block_table(int T, int ord) const {
+    std::vector<uint32_t> inst(count);
+    std::iota(inst.begin(), inst.end(), 0u);
+    if (ord == 1)
+      std::stable_sort(inst.begin(), inst.end(), [&](uint32_t a, uint32_t b) {
+        return v_off[a + 1] - v_off[a] > v_off[b + 1] - v_off[b];
+      });
+    std::vector<uint4> t;
+    for (uint32_t k : inst)
+      for (uint32_t v = v_off[k]; v < v_off[k + 1]; v += (uint32_t)T)
+        t.push_back(make_uint4(k, v, std::min<uint32_t>((uint32_t)T, v_off[k + 1] - v), 0u));
+    return t;
+  }
+
+  void launch(int T, const uint4* blk, uint32_t nblk, hipStream_t s) {
+    const float* in = state[cur].p;
+    float* out = state[cur ^ 1].p;
+    if (T == 64)
+      k_verlet_batch<64><<<nblk, 64, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+    else if (T == 128)
+      k_verlet_batch<128><<<nblk, 128, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+    else
+      k_verlet_batch<256><<<nblk, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+    CC_HIP(hipGetLastError());
+  }
+
+  // Time every (tile, order) candidate over a few steps from the initial
+  // state, keep the fastest, and restore the initial state.
+  void autotune() {
+    const size_t n6 = 6 * (size_t)ntotal, n3 = 3 * (size_t)ntotal;
+    DBuf<float> s0, c0;
+    s0.alloc(n6);
+    c0.alloc(n3);
+    CC_HIP(hipMemcpyAsync(s0.p, state[0].p, n6 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    CC_HIP(hipMemcpyAsync(c0.p, contact.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    hipEvent_t e0, e1;
+    CC_HIP(hipEventCreate(&e0));
+    CC_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int T : TILES)
+      for (int ord = 0; ord < 2; ++ord) {
+        const std::vector<uint4> t = block_table(T, ord);
+        DBuf<uint4> blk;
+        blk.upload(t.data(), t.size(), stream);
+        for (int k = 0; k < 3; ++k) {          // warm-up
+          launch(T, blk.p, (uint32_t)t.size(), stream);
+          cur ^= 1;
+        }
+        CC_HIP(hipEventRecord(e0, stream));
+        for (int k = 0; k < 10; ++k) {
+          launch(T, blk.p, (uint32_t)t.size(), stream);
+          cur ^= 1;
+        }
+        CC_HIP(hipEventRecord(e1, stream));
+        CC_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        CC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          tile = T;
+          order = ord;
+        }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cur = 0;
+    CC_HIP(hipMemcpyAsync(state[0].p, s0.p, n6 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    CC_HIP(hipMemcpyAsync(contact.p, c0.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    const std::vector<uint4> t = block_table(tile, order);
+    blocks.upload(t.data(), t.size(), stream);
+    nblocks = (uint32_t)t.size();
+    CC_HIP(hipStreamSynchronize(stream));
+  }
+
+  hipGraphExec_t capture() {
+    const int cur0 = cur;
+    hipGraph_t graph = nullptr;
+    CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < GRAPH_STEPS_B; ++k) {
+        launch(tile, blocks.p, nblocks, stream);
+        cur ^= 1;
+      }
+    } catch (...) {
+      cur = cur0;
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    cur = cur0;
+    CC_HIP(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t g = nullptr;
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw Error(CC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return g;
+  }
+
+  void step(int nsteps, hipStream_t s) {
+    if (!s) s = stream;
+    while (nsteps > 0) {
+      if (nsteps >= GRAPH_STEPS_B) {
+        hipGraphExec_t& g = graphs[cur];
+        if (!g) g = capture();
+        CC_HIP(hipGraphLaunch(g, s));
+        nsteps -= GRAPH_STEPS_B;             // even: the parity is unchanged
+      } else {
+        launch(tile, blocks.p, nblocks, s);
+        cur ^= 1;
+        --nsteps;
+      }
+    }
+  }
+};
+
+}  // namespace cc
+
+using namespace cc;
+
+struct cc_batch {
+  Batch b;
+};
+
+extern "C" {
+
+int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, const cc_params* params, uint32_t count,
+                    cc_batch** out) {
+  return guard([&] {
+    CC_REQUIRE(cloths && params && out && count >= 1, "cc_batch_create: bad argument");
+    *out = nullptr;
+    int ndev = 0;
+    CC_HIP(hipGetDeviceCount(&ndev));
+    CC_REQUIRE(ndev > 0, "cc_batch_create: no HIP device");
+    std::unique_ptr<cc_batch> h(new cc_batch);
+    Batch& B = h->b;
+    CC_HIP(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    hipStream_t s = B.stream;
+    B.count = count;
+    B.v_off.assign(count + 1, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+      CC_REQUIRE(cloths[k].n >= 1 && cloths[k].verts4, "cc_batch_create: empty cloth");
+      CC_REQUIRE(cloths[k].ntypes >= 1 && cloths[k].ntypes <= 3, "cc_batch_create: ntypes must be 1..3");
+      CC_REQUIRE(params[k].dt > 0.0f && params[k].mass > 0.0f, "cc_batch_create: dt and mass must be positive");
+      B.v_off[k + 1] = B.v_off[k] + cloths[k].n;
+    }
+    B.ntotal = B.v_off[count];
+    CC_REQUIRE(B.ntotal <= END_MASK, "cc_batch_create: too many vertices");
+    // cloth: state, merged springs with global indices
+    std::vector<float> packed(6 * (size_t)B.ntotal);
+    std::vector<uint2> rec_all;
+    B.h_row.assign((size_t)B.ntotal + 1, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+      std::vector<uint32_t> row;
+      std::vector<uint2> rec;
+      merge_springs(&cloths[k], row, rec);
+      std::vector<float> x3;
+      read_vec3_from(cloths[k].verts4, 4, cloths[k].n, x3);
+      const uint32_t o = B.v_off[k];
+      const uint32_t r0 = (uint32_t)rec_all.size();
+      CC_REQUIRE((uint64_t)r0 + rec.size() < ROW_MASK, "cc_batch_create: too many springs");
+      for (uint32_t i = 0; i < cloths[k].n; ++i) {
+        B.h_row[o + i] = r0 + row[i];
+        for (int c = 0; c < 3; ++c) {
+          packed[6 * (size_t)(o + i) + c] = x3[3 * (size_t)i + c];
+          packed[6 * (size_t)(o + i) + 3 + c] = x3[3 * (size_t)i + c];
+        }
+      }
+      for (uint2 r : rec) rec_all.push_back(make_uint2(((r.x & END_MASK) + o) | (r.x & ~END_MASK), r.y));
+    }
+    B.h_row[B.ntotal] = (uint32_t)rec_all.size();
+    B.state[0].upload(packed.data(), packed.size(), s);
+    B.state[1].alloc(packed.size());
+    B.state[1].zero(s);
+    B.contact.alloc(3 * (size_t)B.ntotal);
+    B.contact.zero(s);
+    B.row.upload(B.h_row.data(), B.h_row.size(), s);
+    if (rec_all.empty()) {
+      B.rec.alloc(1);
+      B.rec.zero(s);
+    } else {
+      B.rec.upload(rec_all.data(), rec_all.size(), s);
+    }
+    B.spill.alloc((size_t)STACK_SPILL * B.ntotal);
+    // bodies: each built by the single-body code, then concatenated
+    std::vector<size_t> n_off(count + 1, 0), l_off(count + 1, 0), vb_off(count + 1, 0);
+    std::vector<std::unique_ptr<Body>> built(count);
+    for (uint32_t k = 0; k < count; ++k) {
+      const cc_body_desc* bd = bodies ? &bodies[k] : nullptr;
+      size_t nn = 1, nl = 0, nv = 0;
+      if (bd && bd->mode != CC_BODY_NONE) {
+        CC_REQUIRE(bd->mode == CC_BODY_STATIC, "cc_batch_create: bodies must be static (or none)");
+        CC_REQUIRE(bd->verts && bd->tris && (bd->stride == 3 || bd->stride == 4), "cc_batch_create: bad body");
+        std::vector<float> bv;
+        read_vec3_from(bd->verts, bd->stride, bd->nv, bv);
+        if (bd->extend) {
+          CC_REQUIRE(bd->normals3, "cc_batch_create: body normals required for extend");
+          int rc = cc_mesh_vertex_extend(bv.data(), 3, bd->nv, bd->tris, bd->nf, nullptr, bd->normals3,
+                                         params[k].body_extend);
+          if (rc) throw Error(rc, cc_last_error());
+        }
+        built[k].reset(new Body);
+        built[k]->init(bv.data(), bd->nv, bd->tris, bd->nf, CC_BODY_STATIC, params[k].box_depth, nullptr,
+                       MotionParams(), 1, 1, s);
+        built[k]->build(0, 0, s);
+        nn = std::max<uint32_t>(bd->nf, 2) - 1;
+        nl = bd->nf;
+        nv = bd->nv;
+      }
+      n_off[k + 1] = n_off[k] + nn;
+      l_off[k + 1] = l_off[k] + nl;
+      vb_off[k + 1] = vb_off[k] + nv;
+    }
+    B.nodes.alloc(std::max<size_t>(n_off[count], 1));
+    B.nodes.zero(s);
+    B.leaf_tri.alloc(std::max<size_t>(l_off[count], 1));
+    B.verts.alloc(std::max<size_t>(3 * vb_off[count], 1));
+    B.leaf0.alloc(6 * (size_t)count);
+    B.leaf0.zero(s);
+    std::vector<InstDesc> desc(count);
+    for (uint32_t k = 0; k < count; ++k) {
+      desc[k].a = step_args(params[k], cloths[k].ntypes);
+      BvhView& v = desc[k].bvh;
+      v.nodes = B.nodes.p + n_off[k];
+      v.leaf_tri = B.leaf_tri.p + l_off[k];
+      v.verts = B.verts.p + 3 * vb_off[k];
+      v.leaf0_box = B.leaf0.p + 6 * (size_t)k;
+      v.m = 0;
+      if (built[k]) {
+        Body& bd = *built[k];
+        v.m = bd.m;
+        const Slot& S = bd.slot[0];
+        CC_HIP(hipMemcpyAsync((void*)v.nodes, S.nodes.p, S.nodes.n * sizeof(Node), hipMemcpyDeviceToDevice, s));
+        CC_HIP(hipMemcpyAsync((void*)v.leaf_tri, bd.leaf_tri.p, bd.m * sizeof(uint4), hipMemcpyDeviceToDevice, s));
+        CC_HIP(hipMemcpyAsync((void*)v.verts, S.verts.p, 3 * (size_t)bd.nv * sizeof(float), hipMemcpyDeviceToDevice,
+                              s));
+        CC_HIP(hipMemcpyAsync((void*)v.leaf0_box, S.leaf0_box.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
+      }
+    }
+    B.insts.upload(desc.data(), desc.size(), s);
+    CC_HIP(hipStreamSynchronize(s));
+    built.clear();
+    B.autotune();
+    *out = h.release();
+  });
+}
+
+int cc_batch_destroy(cc_batch* b) {
+  return guard([&] {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->b.stream);
+    delete b;
+  });
+}
+
+int cc_batch_set_pins(cc_batch* b, uint32_t inst, const uint32_t* idx, uint32_t k) {
+  return guard([&] {
+    CC_REQUIRE(b && inst < b->b.count && (idx || k == 0), "cc_batch_set_pins: bad argument");
+    Batch& B = b->b;
+    const uint32_t o = B.v_off[inst], n = B.v_off[inst + 1] - o;
+    for (uint32_t q = 0; q < k; ++q) CC_REQUIRE(idx[q] < n, "cc_batch_set_pins: index out of range");
+    CC_HIP(hipDeviceSynchronize());
+    for (uint32_t i = o; i < o + n; ++i) B.h_row[i] &= ROW_MASK;
+    float rec6[6];
+    for (uint32_t q = 0; q < k; ++q) {
+      B.h_row[o + idx[q]] |= PIN_BIT;
+      float* dst = B.state[B.cur].p + 6 * (size_t)(o + idx[q]);
+      CC_HIP(hipMemcpy(rec6, dst, 24, hipMemcpyDeviceToHost));
+      for (int c = 0; c < 3; ++c) rec6[3 + c] = rec6[c];
+      CC_HIP(hipMemcpy(dst, rec6, 24, hipMemcpyHostToDevice));
+    }
+    CC_HIP(hipMemcpy(B.row.p + o, B.h_row.data() + o, (size_t)n * 4, hipMemcpyHostToDevice));
+  });
+}
+
+int cc_batch_step(cc_batch* b, int nsteps, void* stream) {
+  return guard([&] {
+    CC_REQUIRE(b && nsteps >= 0, "cc_batch_step: bad argument");
+    b->b.step(nsteps, (hipStream_t)stream);
+  });
+}
+
+int cc_batch_synchronize(cc_batch* b) {
+  return guard([&] {
+    CC_REQUIRE(b, "cc_batch_synchronize: null");
+    CC_HIP(hipStreamSynchronize(b->b.stream));
+    CC_HIP(hipDeviceSynchronize());
+  });
+}
+
+int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, float* contact_n) {
+  return guard([&] {
+    CC_REQUIRE(b && inst < b->b.count, "cc_batch_get_state: bad argument");
+    Batch& B = b->b;
+    CC_HIP(hipDeviceSynchronize());
+    const uint32_t o = B.v_off[inst], n = B.v_off[inst + 1] - o;
+    std::vector<float> st(6 * (size_t)n), c(3 * (size_t)n);
+    CC_HIP(hipMemcpy(st.data(), B.state[B.cur].p + 6 * (size_t)o, st.size() * 4, hipMemcpyDeviceToHost));
+    CC_HIP(hipMemcpy(c.data(), B.contact.p + 3 * (size_t)o, c.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        if (x_cur) x_cur[3 * (size_t)i + k] = st[6 * (size_t)i + k];
+        if (x_prev) x_prev[3 * (size_t)i + k] = st[6 * (size_t)i + 3 + k];
+        if (contact_n) contact_n[3 * (size_t)i + k] = c[3 * (size_t)i + k];
+      }
+  });
+}
+
+int cc_batch_info(cc_batch* b, uint32_t* vertices, uint32_t* tile, uint32_t* order) {
+  return guard([&] {
+    CC_REQUIRE(b, "cc_batch_info: null");
+    if (vertices) *vertices = b->b.ntotal;
+    if (tile) *tile = (uint32_t)b->b.tile;
+    if (order) *order = (uint32_t)b->b.order;
+  });
+}
+
+}  // extern "C"

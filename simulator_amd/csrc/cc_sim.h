@@ -262,6 +262,12 @@ struct Sim {
   }
 };
 
+// All spring types of a cloth merged into one CSR row per vertex (types in
+// order, each row in its CSR order); records {end | type << 30, rest bits}.
+void merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec);
+// n vec3 read from an array of stride floats per vertex
+void read_vec3_from(const float* v, uint32_t stride, uint32_t n, std::vector<float>& out);
+
 }  // namespace cc
 
 struct cc_sim {

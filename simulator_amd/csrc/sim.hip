@@ -14,8 +14,7 @@
 #include <memory>
 #include <vector>
 
-#include "cc_device.h"
-#include "cc_sim.h"
+#include "cc_step.h"
 
 namespace cc {
 
@@ -33,92 +32,8 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    // The vertex's own records are loaded together and waited for once: the
-    // compiler otherwise issues each load at its first use, one memory round
-    // trip after another.
-    const uint32_t r0raw = row[i], r1raw = row[i + 1];
-    const float2* s2 = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)i);
-    const float2 e0 = s2[0], e1 = s2[1], e2 = s2[2];
-    const f3 c = ld3(contact + 3 * (size_t)i);
-    asm volatile("" ::"v"(r0raw), "v"(r1raw), "v"(e0.x), "v"(e0.y), "v"(e1.x), "v"(e1.y), "v"(e2.x), "v"(e2.y),
-                 "v"(c.x), "v"(c.y), "v"(c.z));
-    const uint32_t r0 = r0raw & ROW_MASK, r1 = r1raw & ROW_MASK;
-    const f3 p = mk(e0.x, e0.y, e1.x), q = mk(e1.y, e2.x, e2.y);
-    f3 np, nq, nc = mk(0.0f, 0.0f, 0.0f);
-    int32_t hit = -1;
-    bool inside = false;
-    if (r0raw & PIN_BIT) {
-      np = p;
-      nq = p;
-    } else {
-      const f3 vel = mk((p.x - q.x) * a.inv_dt, (p.y - q.y) * a.inv_dt, (p.z - q.z) * a.inv_dt);
-      f3 F = mk(fmaf(vel.x, a.damp, a.gmx), fmaf(vel.y, a.damp, a.gmy), fmaf(vel.z, a.damp, a.gmz));
-      f3 S = mk(0.f, 0.f, 0.f);   // all spring types, in merged-row order
-      // batches of SPRING_BATCH springs: all record loads, then all
-      // neighbour gathers are in flight together
-      // Batches of SPRING_BATCH springs, branch-free: all record loads, then
-      // all neighbour gathers, are issued before any use.  Slots past the
-      // row end gather the vertex itself and contribute exactly +0 (with a
-      // condition per slot the compiler sinks each gather into its branch,
-      // one memory round trip per spring).
-      for (uint32_t k0 = r0; k0 < r1; k0 += SPRING_BATCH) {
-        uint2 rr[SPRING_BATCH];
-        float2 g[SPRING_BATCH][3];
-#pragma unroll
-        for (int q = 0; q < SPRING_BATCH; ++q) {
-          rr[q] = rec[min(k0 + (uint32_t)q, r1 - 1)];
-          if (k0 + q >= r1) rr[q] = make_uint2(i, 0u);
-        }
-#pragma unroll
-        for (int q = 0; q < SPRING_BATCH; ++q) {
-          const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(rr[q].x & END_MASK));
-          g[q][0] = sj[0];
-          g[q][1] = sj[1];
-          g[q][2] = sj[2];
-        }
-#pragma unroll
-        for (int q = 0; q < SPRING_BATCH; ++q) {
-          const uint32_t t = rr[q].x >> 30;
-          const float rest = __uint_as_float(rr[q].y) * a.rest_scale;
-          const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
-          const f3 p2 = mk(g[q][0].x, g[q][0].y, g[q][1].x), q2 = mk(g[q][1].y, g[q][2].x, g[q][2].y);
-          const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
-          const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
-                           fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
-          const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
-          const float rinv = rsqrtf(L2);
-          const float L = L2 * rinv;
-          const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
-          // an empty slot: dP = (eps, eps, eps) > 0 and sc = +0, so the
-          // products are +0 and S (never -0) is unchanged
-          const float sc = (k0 + q < r1) ? fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv : 0.0f;
-          S.x = fmaf(dP.x, sc, S.x);
-          S.y = fmaf(dP.y, sc, S.y);
-          S.z = fmaf(dP.z, sc, S.z);
-        }
-      }
-      F = F + S;
-      const float cf = fmaf(c.z, F.z, fmaf(c.y, F.y, c.x * F.x));
-      F = mk(fmaf(-c.x, cf, F.x), fmaf(-c.y, cf, F.y), fmaf(-c.z, cf, F.z));
-      // pos + pos - pos_old + acc*dt*dt (verlet.cu:155) evaluated as
-      // pos + ((pos - pos_old) + acc*dt^2): pos - pos_old is exact for close
-      // values, so the only rounding at |pos| is the last addition.
-      np = mk(p.x + fmaf(F.x * a.inv_mass, a.dt2, p.x - q.x), p.y + fmaf(F.y * a.inv_mass, a.dt2, p.y - q.y),
-              p.z + fmaf(F.z * a.inv_mass, a.dt2, p.z - q.z));
-      nq = p;
-      if (DEBUG) st3(dbg_pre + 3 * (size_t)i, np);
-      inside = collide_project(bvh, np, nc, hit, visits, Spill{spill + i, n});
-      if (inside) nq = np;
-    }
-    if (DEBUG) {
-      if (r0raw & PIN_BIT) st3(dbg_pre + 3 * (size_t)i, np);
-      dbg_hit[i] = hit;
-      dbg_in[i] = inside ? 1 : 0;
-    }
-    float* o = sout + 6 * (size_t)i;
-    o[0] = np.x; o[1] = np.y; o[2] = np.z;
-    o[3] = nq.x; o[4] = nq.y; o[5] = nq.z;
-    st3(contact + 3 * (size_t)i, nc);
+    step_vertex<DEBUG>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre, dbg_hit,
+                       dbg_in);
   }
   if (COUNT) {
     // block sum into this block's own counter pair (no shared atomics)
@@ -240,7 +155,7 @@ void Sim::launch_step(const float* in, float* out, float* contact_io, const Step
 
 using namespace cc;
 
-static void merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec) {
+void cc::merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec) {
   const uint32_t n = c->n;
   row.assign((size_t)n + 1, 0);
   size_t total = 0;
@@ -267,7 +182,7 @@ static void merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, st
   row[n] = (uint32_t)k;
 }
 
-static void read_vec3_from(const float* v, uint32_t stride, uint32_t n, std::vector<float>& out) {
+void cc::read_vec3_from(const float* v, uint32_t stride, uint32_t n, std::vector<float>& out) {
   out.resize(3 * (size_t)n);
   for (uint32_t i = 0; i < n; ++i)
     for (int c = 0; c < 3; ++c) out[3 * (size_t)i + c] = v[(size_t)stride * i + c];
