@@ -325,7 +325,9 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
    cc_sim handles stepped in turn, bit for bit.  The workgroup size and the
    order of the workgroups are picked at creation by timing each candidate
    (cc_batch_info reports the choice).  bodies may be NULL (no colliders) or
-   hold count entries of mode NONE or STATIC. */
+   hold count entries of any mode: REFIT / REBUILD bodies are posed (and
+   refitted / rebuilt) at every frame for all instances together before the
+   cloths step, as cc_sim_step does for one body. */
 typedef struct cc_batch cc_batch;
 int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, const cc_params* params, uint32_t count,
                     cc_batch** out);
@@ -337,6 +339,12 @@ int cc_batch_synchronize(cc_batch* b);
 /* cc_sim_set_pins / cc_sim_get_state for instance inst. */
 int cc_batch_set_pins(cc_batch* b, uint32_t inst, const uint32_t* idx, uint32_t k);
 int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, float* contact_n);
+/* The body of instance inst at the current frame: vertices (vec3, may be
+   NULL) and the LBVH in the encoding of cc_bvh_export. */
+int cc_batch_body_export(cc_batch* b, uint32_t inst, float* verts3, uint64_t* keys, uint32_t* order,
+                         int32_t* childA, int32_t* childB, int32_t* parent_int, int32_t* parent_leaf,
+                         float* box_int, float* box_leaf, float* root6);
+int cc_batch_frame(cc_batch* b, uint64_t* frame);
 /* Total vertices, the chosen workgroup size (64, 128 or 256) and order
    (0: instance order, 1: largest instance first); any pointer may be NULL. */
 int cc_batch_info(cc_batch* b, uint32_t* vertices, uint32_t* tile, uint32_t* order);

@@ -1,11 +1,13 @@
 """Batched instances over the C ABI (cc_batch_*).
 
 `instances(count, seed)` generates the batched workload: `count` independent
-grid cloths of side log-uniform in [16, 192] over their own static
-capsule-union bodies of 2k-40k triangles, each with spring constants drawn
+grid cloths of side log-uniform in [16, 192] over their own moving
+capsule-union bodies of 2k-40k triangles (6 capsules, rigid sway and
+breathing, LBVH rebuilt every frame), each with spring constants drawn
 within +/-30% of the small workload's, dt = 1e-3.  `Batch(scenes)` advances
-all of them with one launch per step; it equals one `Simulator` per scene
-stepped in turn, bit for bit.
+all of them together (bodies posed and rebuilt in a few launches, then one
+step launch); it equals one `Simulator` per scene stepped in turn, bit for
+bit.
 
 The reference holds a single Simulator per Scene (simulator.h:9-59,
 scene.cpp:176-186); a batch is that object repeated, with the per-vertex step
@@ -17,6 +19,7 @@ import numpy as np
 
 from . import _lib as L
 from . import scene as S
+from .simulator import body_desc
 
 SIDE_RANGE = (16, 192)
 CAPSULE_S = (16, 64)        # capsule tessellation: nseg = s, nring = ncyl = s / 4
@@ -31,8 +34,10 @@ def capsule_triangles(s, ncaps=BATCH_CAPS):
     return ncaps * s * 2 * (2 * nring + ncyl - 1)
 
 
-def instances(count=256, seed=0, steps=300, static=True):
-    """Seeded scenes of the batched workload (see the module docstring)."""
+def instances(count=256, seed=0, steps=300, mode=L.CC_BODY_REBUILD):
+    """Seeded scenes of the batched workload (see the module docstring);
+    `mode` is the bodies' per-frame update (REBUILD: posed and rebuilt every
+    frame, REFIT: posed and refitted, STATIC: fixed)."""
     rng = np.random.default_rng(seed)
     lo, hi = np.log(SIDE_RANGE[0]), np.log(SIDE_RANGE[1])
     out = []
@@ -42,7 +47,7 @@ def instances(count=256, seed=0, steps=300, static=True):
         ks = tuple(float(b * rng.uniform(0.7, 1.3)) for b in K_BASE)
         bseed = int(rng.integers(1, 2**31 - 1))
         v, t, nrm, caps = S.capsule_body(BATCH_CAPS, 0.04, 0.15, *S.CAPSULE_BOX, s, s // 4, s // 4, bseed)
-        body = S.Body(v, t, nrm, L.CC_BODY_STATIC, True, caps, S._motion(bseed))
+        body = S.Body(v, t, nrm, mode, True, caps, S._motion(bseed))
         sc = S.grid_scene("batched", side, 1e-3, steps, body, k=ks, box_depth=0.002, seed=1000 + k)
         out.append(sc)
     return out
@@ -64,26 +69,15 @@ def _cloth_desc(sc, keep):
 
 
 def _body_desc(body, keep):
-    bd = L.cc_body_desc()
     if body is None or body.mode == L.CC_BODY_NONE:
+        bd = L.cc_body_desc()
         bd.mode = L.CC_BODY_NONE
         return bd
-    bv = np.ascontiguousarray(body.verts, np.float32)
-    bt = np.ascontiguousarray(body.tris, np.uint32)
-    bn = np.ascontiguousarray(body.normals, np.float32)
-    keep += [bv, bt, bn]
-    bd.nv, bd.stride, bd.nf = bv.shape[0], 3, bt.shape[0]
-    bd.verts = C.cast(L.ptr(bv), C.POINTER(C.c_float))
-    bd.tris = C.cast(L.ptr(bt), C.POINTER(C.c_uint32))
-    bd.normals3 = C.cast(L.ptr(bn), C.POINTER(C.c_float))
-    bd.mode = body.mode
-    bd.extend = 1 if body.extend else 0
-    return bd
+    return body_desc(body, keep)
 
 
 class Batch:
-    """All scenes advanced together; instance k is scenes[k].  Bodies must be
-    static (or absent)."""
+    """All scenes advanced together; instance k is scenes[k]."""
 
     def __init__(self, scenes):
         lib = L.lib()
@@ -95,6 +89,8 @@ class Batch:
         bds = (L.cc_body_desc * self.count)(*[_body_desc(sc.body, keep) for sc in scenes])
         prm = (L.cc_params * self.count)(*[sc.params for sc in scenes])
         self.sizes = [sc.n for sc in scenes]
+        self._body_m = [0 if sc.body is None else sc.body.tris.shape[0] for sc in scenes]
+        self._body_nv = [0 if sc.body is None else sc.body.verts.shape[0] for sc in scenes]
         h = C.c_void_p()
         L.check(lib.cc_batch_create(cds, bds, prm, self.count, C.byref(h)))
         self._h = h
@@ -140,3 +136,26 @@ class Batch:
         v, t, o = C.c_uint32(), C.c_uint32(), C.c_uint32()
         L.check(L.lib().cc_batch_info(self._h, C.byref(v), C.byref(t), C.byref(o)))
         return v.value, t.value, o.value
+
+    @property
+    def frame(self):
+        f = C.c_uint64()
+        L.check(L.lib().cc_batch_frame(self._h, C.byref(f)))
+        return f.value
+
+    def body_export(self, inst):
+        """(vertices, LBVH dict in the keys of Simulator.bvh_export) of
+        instance inst's body at the current frame."""
+        m, nv = self._body_m[inst], self._body_nv[inst]
+        mi = max(m - 1, 0)
+        verts = np.zeros((nv, 3), np.float32)
+        d = {"keys": np.zeros(m, np.uint64), "order": np.zeros(m, np.uint32),
+             "childA": np.zeros(mi, np.int32), "childB": np.zeros(mi, np.int32),
+             "parent_int": np.zeros(mi, np.int32), "parent_leaf": np.zeros(m, np.int32),
+             "box_int": np.zeros((mi, 6), np.float32), "box_leaf": np.zeros((m, 6), np.float32),
+             "root": np.zeros(6, np.float32)}
+        L.check(L.lib().cc_batch_body_export(self._h, int(inst), L.ptr(verts), L.ptr(d["keys"]), L.ptr(d["order"]),
+                                             L.ptr(d["childA"]), L.ptr(d["childB"]), L.ptr(d["parent_int"]),
+                                             L.ptr(d["parent_leaf"]), L.ptr(d["box_int"]), L.ptr(d["box_leaf"]),
+                                             L.ptr(d["root"])))
+        return verts, d

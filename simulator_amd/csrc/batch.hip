@@ -15,6 +15,7 @@
 #include <memory>
 #include <numeric>
 
+#include "cc_bbody.h"
 #include "cc_step.h"
 
 namespace cc {
@@ -52,11 +53,9 @@ struct Batch {
   DBuf<uint32_t> row;
   DBuf<uint2> rec;
   DBuf<int32_t> spill;
-  // bodies, concatenated
-  DBuf<Node> nodes;
-  DBuf<uint4> leaf_tri;
-  DBuf<float> verts, leaf0;
+  BatchBodies bodies;
   DBuf<InstDesc> insts;
+  uint64_t frame = 0;
   // the chosen launch shape
   int tile = 256, order = 0;
   DBuf<uint4> blocks;
@@ -152,6 +151,7 @@ struct Batch {
     CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
       for (int k = 0; k < GRAPH_STEPS_B; ++k) {
+        bodies.launch(bodies.step, k, stream);
         launch(tile, blocks.p, nblocks, stream);
         cur ^= 1;
       }
@@ -171,18 +171,27 @@ struct Batch {
     return g;
   }
 
+  // Step g: the moving bodies are posed at frame g and rebuilt / refitted,
+  // then every cloth steps against its body (the single simulator's order).
   void step(int nsteps, hipStream_t s) {
     if (!s) s = stream;
     while (nsteps > 0) {
       if (nsteps >= GRAPH_STEPS_B) {
         hipGraphExec_t& g = graphs[cur];
         if (!g) g = capture();
+        bodies.upload_motion(frame, GRAPH_STEPS_B, s);
         CC_HIP(hipGraphLaunch(g, s));
         nsteps -= GRAPH_STEPS_B;             // even: the parity is unchanged
+        frame += GRAPH_STEPS_B;
       } else {
+        if (bodies.moving) {
+          bodies.upload_motion(frame, 1, s);
+          bodies.launch(bodies.step, 0, s);
+        }
         launch(tile, blocks.p, nblocks, s);
         cur ^= 1;
         --nsteps;
+        ++frame;
       }
     }
   }
@@ -256,64 +265,16 @@ int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, con
       B.rec.upload(rec_all.data(), rec_all.size(), s);
     }
     B.spill.alloc((size_t)STACK_SPILL * B.ntotal);
-    // bodies: each built by the single-body code, then concatenated
-    std::vector<size_t> n_off(count + 1, 0), l_off(count + 1, 0), vb_off(count + 1, 0);
-    std::vector<std::unique_ptr<Body>> built(count);
-    for (uint32_t k = 0; k < count; ++k) {
-      const cc_body_desc* bd = bodies ? &bodies[k] : nullptr;
-      size_t nn = 1, nl = 0, nv = 0;
-      if (bd && bd->mode != CC_BODY_NONE) {
-        CC_REQUIRE(bd->mode == CC_BODY_STATIC, "cc_batch_create: bodies must be static (or none)");
-        CC_REQUIRE(bd->verts && bd->tris && (bd->stride == 3 || bd->stride == 4), "cc_batch_create: bad body");
-        std::vector<float> bv;
-        read_vec3_from(bd->verts, bd->stride, bd->nv, bv);
-        if (bd->extend) {
-          CC_REQUIRE(bd->normals3, "cc_batch_create: body normals required for extend");
-          int rc = cc_mesh_vertex_extend(bv.data(), 3, bd->nv, bd->tris, bd->nf, nullptr, bd->normals3,
-                                         params[k].body_extend);
-          if (rc) throw Error(rc, cc_last_error());
-        }
-        built[k].reset(new Body);
-        built[k]->init(bv.data(), bd->nv, bd->tris, bd->nf, CC_BODY_STATIC, params[k].box_depth, nullptr,
-                       MotionParams(), 1, 1, s);
-        built[k]->build(0, 0, s);
-        nn = std::max<uint32_t>(bd->nf, 2) - 1;
-        nl = bd->nf;
-        nv = bd->nv;
-      }
-      n_off[k + 1] = n_off[k] + nn;
-      l_off[k + 1] = l_off[k] + nl;
-      vb_off[k + 1] = vb_off[k] + nv;
-    }
-    B.nodes.alloc(std::max<size_t>(n_off[count], 1));
-    B.nodes.zero(s);
-    B.leaf_tri.alloc(std::max<size_t>(l_off[count], 1));
-    B.verts.alloc(std::max<size_t>(3 * vb_off[count], 1));
-    B.leaf0.alloc(6 * (size_t)count);
-    B.leaf0.zero(s);
+    // bodies: concatenated, frame 0 built for all of them
+    B.bodies.init(bodies, params, count, s);
     std::vector<InstDesc> desc(count);
     for (uint32_t k = 0; k < count; ++k) {
       desc[k].a = step_args(params[k], cloths[k].ntypes);
-      BvhView& v = desc[k].bvh;
-      v.nodes = B.nodes.p + n_off[k];
-      v.leaf_tri = B.leaf_tri.p + l_off[k];
-      v.verts = B.verts.p + 3 * vb_off[k];
-      v.leaf0_box = B.leaf0.p + 6 * (size_t)k;
-      v.m = 0;
-      if (built[k]) {
-        Body& bd = *built[k];
-        v.m = bd.m;
-        const Slot& S = bd.slot[0];
-        CC_HIP(hipMemcpyAsync((void*)v.nodes, S.nodes.p, S.nodes.n * sizeof(Node), hipMemcpyDeviceToDevice, s));
-        CC_HIP(hipMemcpyAsync((void*)v.leaf_tri, bd.leaf_tri.p, bd.m * sizeof(uint4), hipMemcpyDeviceToDevice, s));
-        CC_HIP(hipMemcpyAsync((void*)v.verts, S.verts.p, 3 * (size_t)bd.nv * sizeof(float), hipMemcpyDeviceToDevice,
-                              s));
-        CC_HIP(hipMemcpyAsync((void*)v.leaf0_box, S.leaf0_box.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
-      }
+      desc[k].bvh = BvhView{nullptr, nullptr, nullptr, nullptr, 0};
     }
+    for (uint32_t j = 0; j < B.bodies.nb; ++j) desc[B.bodies.inst[j]].bvh = B.bodies.view(j);
     B.insts.upload(desc.data(), desc.size(), s);
     CC_HIP(hipStreamSynchronize(s));
-    built.clear();
     B.autotune();
     *out = h.release();
   });
@@ -377,6 +338,33 @@ int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, 
         if (x_prev) x_prev[3 * (size_t)i + k] = st[6 * (size_t)i + 3 + k];
         if (contact_n) contact_n[3 * (size_t)i + k] = c[3 * (size_t)i + k];
       }
+  });
+}
+
+int cc_batch_body_export(cc_batch* b, uint32_t inst, float* verts3, uint64_t* keys, uint32_t* order,
+                         int32_t* childA, int32_t* childB, int32_t* parent_int, int32_t* parent_leaf,
+                         float* box_int, float* box_leaf, float* root6) {
+  return guard([&] {
+    CC_REQUIRE(b && inst < b->b.count, "cc_batch_body_export: bad argument");
+    Batch& B = b->b;
+    BatchBodies& Y = B.bodies;
+    uint32_t j = 0;
+    while (j < Y.nb && Y.inst[j] != inst) ++j;
+    CC_REQUIRE(j < Y.nb, "cc_batch_body_export: instance has no body");
+    CC_HIP(hipDeviceSynchronize());
+    if (Y.moving) {                 // the pose of the current frame (the next step poses it again)
+      Y.upload_motion(B.frame, 1, B.stream);
+      Y.launch(Y.step, 0, B.stream);
+    }
+    Y.export_tree(j, verts3, keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
+                  B.stream);
+  });
+}
+
+int cc_batch_frame(cc_batch* b, uint64_t* frame) {
+  return guard([&] {
+    CC_REQUIRE(b && frame, "cc_batch_frame: bad argument");
+    *frame = b->b.frame;
   });
 }
 

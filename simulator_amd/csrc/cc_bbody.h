@@ -1,0 +1,103 @@
+// cc_bbody.h -- the collider bodies of a batch: every instance's body in
+// concatenated arrays, posed and rebuilt (or refitted) for all instances
+// together, a handful of launches per frame whatever the instance count.
+#pragma once
+
+#include <vector>
+
+#include "cc_body.h"
+
+namespace cc {
+
+constexpr int BB_MF = 12 + MAX_CAPS;   // floats per motion frame and body
+constexpr int BB_FRAMES = 16;          // motion frames held on the device (one step graph)
+
+// One body of the batch; offsets index the concatenated arrays.
+struct BodyDesc {
+  uint32_t nv, m, ncaps, mode;
+  uint32_t v0;    // first vertex
+  uint32_t t0;    // first triangle == first leaf / key
+  uint32_t n0;    // first internal node
+  uint32_t c0;    // first capsule (p0 / p1 entries)
+  uint32_t cv0;   // first capsule vertex-range entry (ncaps + 1 per body)
+  uint32_t p0;    // first partial root box (one per 256 triangles)
+  float depth;
+  uint32_t pad;
+};
+
+// A node finished by one level of the bottom-up build and handed to the
+// next (its sorted-leaf range, Karras index and box).
+struct alignas(16) BItem {
+  int32_t l, r, idx, pad;
+  float box[6];
+  float pad2[2];
+};
+
+// One workgroup of a build level: chunk q of the body's level, input slots
+// from in_base (n_in of them; leaves at level 0), output slots at out_base.
+struct LvBlock {
+  uint32_t body, q, in_base, out_base, n_in, pad[3];
+};
+
+constexpr int BB_MAX_LEVELS = 5;       // 2^19 leaves: 512 -> 64 -> 8 -> 1 chunks
+
+// Work lists of one kind of pass: block tables {body, first, count, aux} of
+// the grid-wide kernels, the body list of the per-body sort, and the
+// workgroups of each build level.
+struct BodyPass {
+  DBuf<uint4> deform, boxes;
+  DBuf<uint32_t> sort;
+  DBuf<LvBlock> level[BB_MAX_LEVELS];
+  uint32_t n_deform = 0, n_boxes = 0, n_sort = 0, n_level[BB_MAX_LEVELS] = {};
+};
+
+struct BatchBodies {
+  uint32_t nb = 0;                    // bodies
+  std::vector<BodyDesc> h;
+  std::vector<uint32_t> inst;         // body -> instance
+  std::vector<MotionParams> mp;
+  std::vector<std::vector<float>> phase;
+  bool moving = false;                // any REFIT / REBUILD body
+
+  DBuf<BodyDesc> desc;
+  DBuf<float> rest, verts;            // [3 * sum nv]
+  DBuf<uint32_t> tris;                // [3 * sum m], body-local vertex indices
+  DBuf<uint32_t> capv;                // [sum (ncaps + 1)]
+  DBuf<float> capp0, capp1;           // [3 * sum ncaps]
+  DBuf<float> boxes;                  // [6 * sum m] expanded box of each triangle
+  DBuf<float> partial;                // [6 * sum ceil(m / 256)]
+  DBuf<uint2> kv[2];                  // [sum m] (code, primitive) sort ping-pong
+  DBuf<uint64_t> keys;                // [sum m]
+  DBuf<BItem> items[2];               // [level-0 chunks * 128] unfinished nodes between levels
+  DBuf<uint32_t> nitems[2];           // [level-0 chunks] their count per chunk
+  DBuf<uint4> leaf_tri;               // [sum m]
+  DBuf<int32_t> leaf_parent;          // [sum m]
+  DBuf<Node> nodes;                   // [sum max(m - 1, 1)]
+  DBuf<float> root, leaf0;            // [6 * nb]
+  DBuf<float> motion;                 // [BB_FRAMES * nb * BB_MF]
+  float* h_motion[2] = {nullptr, nullptr};   // pinned staging of the motion uploads
+  hipEvent_t ev_motion[2] = {nullptr, nullptr};
+  int hm = 0;
+
+  BodyPass all;    // set-up: every body built from its frame-0 pose
+  BodyPass step;   // per frame: moving bodies posed, REBUILD ones re-sorted
+
+  BatchBodies() = default;
+  BatchBodies(const BatchBodies&) = delete;
+  BatchBodies& operator=(const BatchBodies&) = delete;
+  ~BatchBodies();
+
+  // bodies[k] of instance k (NULL or mode NONE: no body); extends the
+  // vertices when asked, checks every layout, uploads, builds frame 0.
+  void init(const cc_body_desc* bodies, const cc_params* params, uint32_t count, hipStream_t s);
+  BvhView view(uint32_t j) const;
+  // motion of frames [frame0, frame0 + nframes) into the device table
+  void upload_motion(uint64_t frame0, int nframes, hipStream_t s);
+  // one pass; k selects the motion frame in the table
+  void launch(const BodyPass& p, int k, hipStream_t s);
+  void export_tree(uint32_t j, float* verts3, uint64_t* keys, uint32_t* order, int32_t* childA, int32_t* childB,
+                   int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6,
+                   hipStream_t s);
+};
+
+}  // namespace cc

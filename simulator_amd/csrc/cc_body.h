@@ -74,6 +74,13 @@ struct Lane {
   void alloc(uint32_t m, bool full_lane, hipStream_t s);
 };
 
+// Host copy of a built tree in the reference encoding (see cc_bvh_export):
+// sorted keys, leaf order, links, child boxes, root box.
+void export_tree_raw(uint32_t m, const uint64_t* keys_d, const uint4* leaf_tri_d, const int32_t* leaf_parent_d,
+                     const Node* nodes_d, const float* root_d, const float* leaf0_d, uint64_t* okeys,
+                     uint32_t* order, int32_t* childA, int32_t* childB, int32_t* parent_int, int32_t* parent_leaf,
+                     float* box_int, float* box_leaf, float* root6, hipStream_t s);
+
 struct Body {
   uint32_t nv = 0;      // body vertices
   uint32_t m = 0;       // triangles == LBVH leaves

@@ -12,7 +12,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "cc_body.h"
+#include "cc_lbvh.h"
 
 namespace cc {
 
@@ -57,37 +57,13 @@ __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, 
   __syncthreads();
   const uint32_t v = blockIdx.x * blockDim.x + tid;
   if (v >= nv) return;
-  // capsule of v: last k with vstart[k] <= v
-  uint32_t lo = 0, hi = ncaps - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (sV[mid] <= v) lo = mid; else hi = mid - 1;
-  }
-  const uint32_t k = lo;
-  const float* R = sP;
-  const float* T = sP + 9;
-  const f3 X = ld3(rest + 3 * (size_t)v);
-  const f3 P0 = ld3(sP0 + 3 * k), P1 = ld3(sP1 + 3 * k);
-  const f3 d = P1 - P0;
-  const float dd = dot(d, d);
-  float tt = 0.0f;
-  if (dd > 0.0f) tt = dot(X - P0, d) / dd;
-  tt = fminf(fmaxf(tt, 0.0f), 1.0f);
-  const f3 A = P0 + d * tt;
-  const f3 Y = A + (X - A) * sP[12 + k];
-  float* o = out + 3 * (size_t)v;
-  o[0] = ((R[0] * Y.x + R[1] * Y.y) + R[2] * Y.z) + T[0];
-  o[1] = ((R[3] * Y.x + R[4] * Y.y) + R[5] * Y.z) + T[1];
-  o[2] = ((R[6] * Y.x + R[7] * Y.y) + R[8] * Y.z) + T[2];
+  const uint32_t k = capsule_of(sV, ncaps, v);
+  deform_point(ld3(rest + 3 * (size_t)v), ld3(sP0 + 3 * k), ld3(sP1 + 3 * k), sP[12 + k], sP, sP + 9,
+               out + 3 * (size_t)v);
 }
 
 // ------------------------------------------------------------ boxes + codes
 constexpr int BOX_THREADS = 256;
-
-__device__ inline void box_sentinel(float* mn, float* mx) {   // bbox.h:17-22
-  mn[0] = mn[1] = mn[2] = 100.0f;
-  mx[0] = mx[1] = mx[2] = -100.0f;
-}
 
 // expanded box per triangle + one partial root box per block
 __global__ __launch_bounds__(BOX_THREADS) void k_prim_boxes(const float* __restrict__ verts,
@@ -268,46 +244,13 @@ __global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict_
 }
 
 // ------------------------------------------------------------ Karras build
-// processInternalNode / determineRange / findSplit (cudaBRT.cu:9-161) with
-// delta = clz64 of 64-bit keys (-1 outside [0, m)).
 __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ keys, int32_t m,
                                                 Node* __restrict__ nodes, int32_t* __restrict__ leaf_parent,
                                                 uint32_t* __restrict__ top_count) {
   const int32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m - 1) return;
-  const uint64_t ki = keys[i];
-  auto delta = [&](int64_t j) -> int {
-    if (j < 0 || j >= m) return -1;
-    return __clzll(ki ^ keys[j]);
-  };
-  const int d = (delta(i + 1) - delta(i - 1)) > 0 ? 1 : -1;
-  const int dmin = delta(i - d);
-  int64_t lmax = 2;
-  while (delta(i + lmax * d) > dmin) lmax *= 2;
-  int64_t l = 0;
-  for (int64_t t = lmax / 2; t >= 1; t /= 2)
-    if (delta(i + (l + t) * d) > dmin) l += t;
-  const int32_t j = (int32_t)(i + l * d);
-  const int32_t first = min(i, j), last = max(i, j);
-  const uint64_t kf = keys[first];
-  const int common = __clzll(kf ^ keys[last]);
-  int32_t split = first, step = last - first;
-  do {
-    step = (step + 1) >> 1;
-    const int32_t ns = split + step;
-    if (ns < last && (kf >> (63 - common)) == (keys[ns] >> (63 - common))) split = ns;
-  } while (step > 1);
-  const int32_t ca = (split == first) ? ~split : split;
-  const int32_t cb = (split + 1 == last) ? ~(split + 1) : split + 1;
-  nodes[i].childA = ca;
-  nodes[i].childB = cb;
-  nodes[i].other = j;
-  if (ca < 0) leaf_parent[split] = i; else nodes[split].parent = i;
-  if (cb < 0) leaf_parent[split + 1] = i; else nodes[split + 1].parent = i;
-  if (i == 0) {
-    nodes[0].parent = -1;
-    *top_count = 0;      // the refit top-tree list is rebuilt after the topology
-  }
+  karras_node(keys, m, i, nodes, leaf_parent);
+  if (i == 0) *top_count = 0;      // the refit top-tree list is rebuilt after the topology
 }
 
 // ------------------------------------------------------------ host side
@@ -455,24 +398,22 @@ void Body::update(int k, int l, uint64_t frame, hipStream_t s) {
   }
 }
 
-void Body::export_tree(int k, uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB,
-                       int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6,
-                       hipStream_t s) {
-  CC_REQUIRE(mode != CC_BODY_NONE, "bvh export: no body");
-  const Lane& L = lane[0];
-  const Slot& S = slot[k];
+void export_tree_raw(uint32_t m, const uint64_t* keys_d, const uint4* leaf_tri_d, const int32_t* leaf_parent_d,
+                     const Node* nodes_d, const float* root_d, const float* leaf0_d, uint64_t* okeys,
+                     uint32_t* order, int32_t* childA, int32_t* childB, int32_t* parent_int, int32_t* parent_leaf,
+                     float* box_int, float* box_leaf, float* root6, hipStream_t s) {
   std::vector<uint64_t> kk(m);
   std::vector<uint4> lt(m);
   std::vector<int32_t> lp(m);
   std::vector<Node> nd(m >= 2 ? m - 1 : 0);
   float r[6], l0[6];
-  CC_HIP(hipMemcpyAsync(kk.data(), L.keys.p, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri_of(k), m * sizeof(uint4), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(lp.data(), L.leaf_parent.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(kk.data(), keys_d, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lt.data(), leaf_tri_d, m * sizeof(uint4), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(lp.data(), leaf_parent_d, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   if (!nd.empty())
-    CC_HIP(hipMemcpyAsync(nd.data(), S.nodes.p, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(r, S.root.p, sizeof(r), hipMemcpyDeviceToHost, s));
-  CC_HIP(hipMemcpyAsync(l0, S.leaf0_box.p, sizeof(l0), hipMemcpyDeviceToHost, s));
+    CC_HIP(hipMemcpyAsync(nd.data(), nodes_d, nd.size() * sizeof(Node), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(r, root_d, sizeof(r), hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(l0, leaf0_d, sizeof(l0), hipMemcpyDeviceToHost, s));
   CC_HIP(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < m; ++i) {
     if (okeys) okeys[i] = kk[i];
@@ -503,6 +444,16 @@ void Body::export_tree(int k, uint64_t* okeys, uint32_t* order, int32_t* childA,
     if (parent_leaf) parent_leaf[l] = lp[l] + 1;
     if (box_leaf) slot_of_child(lp[l], ~(int32_t)l, box_leaf + 6 * (size_t)l);
   }
+}
+
+void Body::export_tree(int k, uint64_t* okeys, uint32_t* order, int32_t* childA, int32_t* childB,
+                       int32_t* parent_int, int32_t* parent_leaf, float* box_int, float* box_leaf, float* root6,
+                       hipStream_t s) {
+  CC_REQUIRE(mode != CC_BODY_NONE, "bvh export: no body");
+  const Lane& L = lane[0];
+  const Slot& S = slot[k];
+  export_tree_raw(m, L.keys.p, leaf_tri_of(k), L.leaf_parent.p, S.nodes.p, S.root.p, S.leaf0_box.p, okeys, order,
+                  childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6, s);
 }
 
 }  // namespace cc

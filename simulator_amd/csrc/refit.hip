@@ -21,7 +21,7 @@
 //    own L2); the kernel boundary between the passes gives the visibility.
 //  * All boxes go to their parent's child slot (the layout the traversal
 //    reads).  fmin/fmax unions are exact, so the order does not matter.
-#include "cc_body.h"
+#include "cc_lbvh.h"
 
 namespace cc {
 
@@ -36,18 +36,6 @@ constexpr int TOP_MAX_LEVELS = 64;                                  // top-tree 
 constexpr uint8_t BELOW_TOP = 0x80;                                 // flag: parent is a top node
 constexpr uint8_t IS_A = 0x40;                                      // flag: child A of the parent
 constexpr uint8_t H_MASK = 0x3f;                                    // in-chunk height (depth < 64)
-
-__device__ inline void put6(float* d, const float* mn, const float* mx) {
-  d[0] = mn[0]; d[1] = mn[1]; d[2] = mn[2];
-  d[3] = mx[0]; d[4] = mx[1]; d[5] = mx[2];
-}
-
-__device__ inline void unite(float* mn, float* mx, const float* b6) {
-  for (int c = 0; c < 3; ++c) {
-    mn[c] = fminf(mn[c], b6[c]);
-    mx[c] = fmaxf(mx[c], b6[3 + c]);
-  }
-}
 
 // ======================================================== topology setup
 // Heights are found with an arrival climb (run once per topology, so its

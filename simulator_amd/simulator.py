@@ -59,6 +59,41 @@ def _params_copy(p):
     return q
 
 
+def body_desc(body, keep):
+    """cc_body_desc of a scene.Body; the arrays it points to are appended to
+    `keep`, which must outlive the create call."""
+    bv = np.ascontiguousarray(body.verts, np.float32)
+    bt = np.ascontiguousarray(body.tris, np.uint32)
+    bn = np.ascontiguousarray(body.normals, np.float32)
+    keep += [bv, bt, bn]
+    bd = L.cc_body_desc()
+    bd.nv, bd.stride, bd.nf = bv.shape[0], 3, bt.shape[0]
+    bd.verts = C.cast(L.ptr(bv), C.POINTER(C.c_float))
+    bd.tris = C.cast(L.ptr(bt), C.POINTER(C.c_uint32))
+    bd.normals3 = C.cast(L.ptr(bn), C.POINTER(C.c_float))
+    bd.mode = body.mode
+    bd.extend = 1 if body.extend else 0
+    if body.caps is not None:
+        caps = {k: np.ascontiguousarray(a) for k, a in body.caps.items()}
+        keep.append(caps)
+        cs = L.cc_capsules(len(caps["radius"]),
+                           *(C.cast(L.ptr(caps[k]), C.POINTER(T)) for k, T in
+                             (("vstart", C.c_uint32), ("p0", C.c_float), ("p1", C.c_float),
+                              ("radius", C.c_float), ("phase", C.c_float))))
+        keep.append(cs)
+        bd.caps = C.pointer(cs)
+    m = body.motion or {}
+    bd.amp = m.get("amp", 0.0)
+    bd.freq = m.get("freq", 0.0)
+    bd.rot_amp = m.get("rot_amp", 0.0)
+    bd.breath_amp = m.get("breath_amp", 0.0)
+    bd.breath_freq = m.get("breath_freq", 0.0)
+    for c in range(3):
+        bd.dir[c] = m.get("dir", (0, 0, 0))[c]
+        bd.pivot[c] = m.get("pivot", (0, 0, 0))[c]
+    return bd
+
+
 class Simulator:
     """Simulator(Mesh& cloth, Mesh& body) (simulation.cu:47-52).
 
@@ -87,35 +122,7 @@ class Simulator:
         bd_ptr = None
         keep = []
         if body is not None and body.mode != L.CC_BODY_NONE:
-            bv = np.ascontiguousarray(body.verts, np.float32)
-            bt = np.ascontiguousarray(body.tris, np.uint32)
-            bn = np.ascontiguousarray(body.normals, np.float32)
-            keep += [bv, bt, bn]
-            bd = L.cc_body_desc()
-            bd.nv, bd.stride, bd.nf = bv.shape[0], 3, bt.shape[0]
-            bd.verts = C.cast(L.ptr(bv), C.POINTER(C.c_float))
-            bd.tris = C.cast(L.ptr(bt), C.POINTER(C.c_uint32))
-            bd.normals3 = C.cast(L.ptr(bn), C.POINTER(C.c_float))
-            bd.mode = body.mode
-            bd.extend = 1 if body.extend else 0
-            if body.caps is not None:
-                caps = {k: np.ascontiguousarray(a) for k, a in body.caps.items()}
-                keep.append(caps)
-                cs = L.cc_capsules(len(caps["radius"]),
-                                   *(C.cast(L.ptr(caps[k]), C.POINTER(T)) for k, T in
-                                     (("vstart", C.c_uint32), ("p0", C.c_float), ("p1", C.c_float),
-                                      ("radius", C.c_float), ("phase", C.c_float))))
-                keep.append(cs)
-                bd.caps = C.pointer(cs)
-            m = body.motion or {}
-            bd.amp = m.get("amp", 0.0)
-            bd.freq = m.get("freq", 0.0)
-            bd.rot_amp = m.get("rot_amp", 0.0)
-            bd.breath_amp = m.get("breath_amp", 0.0)
-            bd.breath_freq = m.get("breath_freq", 0.0)
-            for c in range(3):
-                bd.dir[c] = m.get("dir", (0, 0, 0))[c]
-                bd.pivot[c] = m.get("pivot", (0, 0, 0))[c]
+            bd = body_desc(body, keep)
             bd_ptr = C.byref(bd)
         h = C.c_void_p()
         L.check(lib.cc_sim_create(C.byref(cd), bd_ptr, C.byref(self.params), C.byref(h)))

@@ -1,24 +1,39 @@
 """Batched instances on the GPU: every instance of a batch equals its own
-Simulator bit for bit (same step, same body LBVH, same pins), and the
-batched step stays within the per-step tolerance of the FP64 oracle."""
+Simulator bit for bit (same step, same body pose and LBVH every frame, same
+pins), the batched body build equals the oracle's LBVH, and the batched step
+stays within the per-step tolerance of the FP64 oracle."""
 import numpy as np
 import pytest
 
 import oracle as O
+from simulator_amd import _lib as L
 from simulator_amd import batch as B
 from simulator_amd.simulator import Simulator
 
-from helpers import extend_like_library, say
+from helpers import BVH_FIELDS, assert_bvh_equal, say
 
 pytestmark = pytest.mark.gpu
 
 TOL_STEP = 1e-6       # cloth extent is 1 m
+MODES = {"static": [L.CC_BODY_STATIC], "refit": [L.CC_BODY_REFIT], "rebuild": [L.CC_BODY_REBUILD],
+         "mixed": [L.CC_BODY_REBUILD, L.CC_BODY_REFIT, L.CC_BODY_STATIC, None]}
 
 
-def test_batch_equals_simulators(gpu):
-    scs = B.instances(8, seed=11)
-    scs[3].body = None                                   # an instance without a collider
-    say("[batch] 8 instances, sides", [sc.nx for sc in scs])
+def scenes(count, seed, kind):
+    scs = B.instances(count, seed=seed)
+    for k, sc in enumerate(scs):
+        mode = MODES[kind][k % len(MODES[kind])]
+        if mode is None:
+            sc.body = None
+        else:
+            sc.body.mode = mode
+    return scs
+
+
+@pytest.mark.parametrize("kind", list(MODES))
+def test_batch_equals_simulators(gpu, kind):
+    scs = scenes(8, 11, kind)
+    say(f"[batch] {kind}: 8 instances, sides", [sc.nx for sc in scs])
     b = B.Batch(scs)
     nv, tile, order = b.info()
     say(f"[batch] {nv} vertices, tile {tile}, order {order}")
@@ -37,11 +52,55 @@ def test_batch_equals_simulators(gpu):
             got = b.get_state(k)
             for g, w, name in zip(got, want, ("x_cur", "x_prev", "contact")):
                 np.testing.assert_array_equal(g, w, err_msg=f"instance {k} {name} after {done} steps")
+    assert b.frame == done
     touched = [int((np.abs(b.get_state(k)[2]).sum(axis=1) > 0).sum()) for k in range(len(scs))]
     say(f"[batch] contacts per instance after {done} steps: {touched}")
-    assert touched[3] == 0 and sum(touched) > 1000
+    assert sum(touched) > 1000
     for k, sc in enumerate(scs):
+        if sc.body is None:
+            assert touched[k] == 0
         np.testing.assert_array_equal(b.get_state(k)[0][sc.pins], sc.pos[sc.pins])
+    for s in sims:
+        s.close()
+    b.close()
+
+
+def refit_oracle(d0, verts, depth):
+    """Topology of the frame-0 build, boxes recomputed from `verts`."""
+    d = dict(d0)
+    lb = O.prim_boxes(verts, d0["leaf_tri"], depth)
+    d["box_leaf"] = lb
+    d["box_int"] = O.refit(d0["childA"], d0["childB"], d0["parent_int"], lb)
+    return d
+
+
+def test_batch_body_trees(gpu):
+    """Pose and LBVH of every body at several frames: equal to the single
+    simulator's, and to the oracle's build (REBUILD, STATIC) or refit of the
+    frame-0 topology (REFIT)."""
+    scs = scenes(4, 23, "mixed")[:3]                     # rebuild, refit, static
+    b = B.Batch(scs)
+    sims = [Simulator.from_scene(sc) for sc in scs]
+    d0 = {}
+    for frame in (0, 1, 17, 40):
+        if frame:
+            step = frame - b.frame
+            b.simulate(step)
+            for s in sims:
+                s.simulate(step)
+            b.synchronize()
+        for k, sc in enumerate(scs):
+            say(f"[batch] body tree, frame {frame}, instance {k} mode {sc.body.mode}, {sc.body.tris.shape[0]} triangles")
+            v, t = b.body_export(k)
+            np.testing.assert_array_equal(v, sims[k].body_vertices())
+            assert_bvh_equal(t, sims[k].bvh_export())
+            np.testing.assert_array_equal(t["root"], sims[k].bvh_export()["root"])
+            want = O.bvh_build(v, sc.body.tris, sc.params.box_depth)
+            if frame == 0:
+                d0[k] = want
+            if sc.body.mode == L.CC_BODY_REFIT:
+                want = refit_oracle(d0[k], v, sc.params.box_depth)
+            assert_bvh_equal(t, want)
     for s in sims:
         s.close()
     b.close()
@@ -50,29 +109,29 @@ def test_batch_equals_simulators(gpu):
 def test_batch_step_vs_fp64_oracle(gpu):
     scs = B.instances(4, seed=5)
     b = B.Batch(scs)
-    say("[batch] per-step FP64 agreement, sides", [sc.nx for sc in scs])
-    setups = []
+    say("[batch] per-step FP64 agreement, rebuilt bodies, sides", [sc.nx for sc in scs])
+    pins = []
     for sc in scs:
-        ext = extend_like_library(sc.body.verts, sc.body.tris, sc.body.normals, sc.params.body_extend)
-        h = O.BvhHandle(O.bvh_build(ext, sc.body.tris, sc.params.box_depth))
-        pins = np.zeros(sc.n, np.uint8)
-        pins[sc.pins] = 1
-        setups.append((h, O.params_from(sc.params), pins))
+        p = np.zeros(sc.n, np.uint8)
+        p[sc.pins] = 1
+        pins.append(p)
     worst, nflips = 0.0, 0
     for k in range(0, 300, 30):
         b.simulate(30 if k else 1)
         before = [b.get_state(i) for i in range(len(scs))]
+        bodies = [b.body_export(i)[0] for i in range(len(scs))]   # the pose of the coming step
         b.simulate(1)
         b.synchronize()
         for i, sc in enumerate(scs):
-            h, prm, pins = setups[i]
+            h = O.BvhHandle(O.bvh_build(bodies[i], sc.body.tris, sc.params.box_depth))
             x, xp, c = before[i]
-            x64, _, _, ins64, _ = O.step_f64(x, xp, c, sc.springs, prm, pins, h, with_hit=True)
-            xg = b.get_state(i)[0]
+            x64, _, _, ins64, _ = O.step_f64(x, xp, c, sc.springs, O.params_from(sc.params), pins[i], h,
+                                             with_hit=True)
+            xg, _, cg = b.get_state(i)
             # a vertex resting on the body (plane distance ~0) can test inside
             # in FP32 and outside in FP64 or the other way; those flips are
             # excluded from the tolerance and bounded below
-            ins32 = (np.abs(b.get_state(i)[2]).sum(axis=1) > 0)
+            ins32 = np.abs(cg).sum(axis=1) > 0
             same = ins32 == (ins64 == 1)
             d = np.abs(xg - x64).max(axis=1)
             worst = max(worst, float(d[same].max()))
