@@ -4,9 +4,12 @@ Workload (N=1): the large configuration -- a 512x512 cloth (262,144 vertices,
 3,135,492 directed springs of three types) over a 12-capsule body of 501,120
 triangles that is deformed every frame with its LBVH refitted, FP32, dt=5e-4.
 One step = body deformation + LBVH refit + fused Verlet/spring/collision
-kernel.  Prints ONE JSON line.
+kernel.  Prints ONE JSON line.  Beside the headline: "batched" (256 uneven
+instances with bodies rebuilt every frame, through one Batch and through the
+plain per-instance loop) and "adjoint" (forward + backward of a 200-step
+differentiable rollout against the plain rollout).
 
-  python bench.py [--gpus 1] [--steps 500] [--warmup 50] [--no-cpu]
+  python bench.py [--gpus 1] [--steps 500] [--warmup 50] [--no-cpu] [--no-batch] [--no-adjoint]
 """
 import argparse
 import json
@@ -115,6 +118,89 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1):
                       f"{el:.1f} s"}
 
 
+def batched_leg(warmup=16, steps=160, plain_steps=32):
+    """The batched workload (256 instances, bodies rebuilt every frame) through
+    one Batch, beside the plain per-instance loop (one Simulator per instance
+    advanced in turn, no synchronisation between instances) on the same GPU."""
+    from simulator_amd import batch as B
+    from simulator_amd.simulator import Simulator
+    scs = B.instances(256, seed=0)
+    n = sum(sc.n for sc in scs)
+    b = B.Batch(scs)
+    b.simulate(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.simulate(steps)
+    b.synchronize()
+    tb = time.perf_counter() - t0
+    nv, tile, order = b.info()
+    b.close()
+    sims = [Simulator.from_scene(sc) for sc in scs]
+    for s in sims:
+        s.simulate(16)
+    for s in sims:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.simulate(plain_steps)
+    for s in sims:
+        s.synchronize()
+    tp = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    vb, vp = n * steps / tb, n * plain_steps / tp
+    return {"workload": "256 instances, cloth side log-uniform [16,192] (seed 0), capsule bodies of "
+                        "2,112-36,096 triangles rebuilt every frame, dt=1e-3",
+            "instances": len(scs), "vertices": n, "body_triangles": int(sum(sc.body.tris.shape[0] for sc in scs)),
+            "value": vb, "unit": "vertex-steps/s", "ms_per_step": 1e3 * tb / steps, "steps": steps,
+            "plain_loop_value": vp, "plain_loop_ms_per_step": 1e3 * tp / plain_steps,
+            "plain_loop_steps": plain_steps, "speedup_vs_plain": vb / vp,
+            "tile": tile, "order": "largest-first" if order else "instance"}
+
+
+def adjoint_leg(nsteps=200, reps=3):
+    """Forward + backward of the differentiable rollout (128x128 cloth over the
+    icosphere, 200 steps, checkpoints every 20) beside the plain 200-step
+    simulation of the same scene."""
+    from simulator_amd import scene as S
+    from simulator_amd.adjoint import Adjoint
+    from simulator_amd.simulator import Simulator
+    sc = S.adjoint_scene(steps=nsteps)
+    sim = Simulator.from_scene(sc)
+    adj = Adjoint(sim, nsteps, 20)
+    x0 = sc.pos.astype(np.float32)
+    v0 = np.zeros_like(x0)
+    theta = np.array([sc.params.k[0], sc.params.k[1], sc.params.k[2], sc.params.damp, sc.params.rest_scale])
+    g = np.ones_like(x0)
+    adj.forward(x0, v0, theta)
+    adj.backward(g)
+    tf = tbw = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        adj.forward(x0, v0, theta)
+        t1 = time.perf_counter()
+        adj.backward(g)
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tbw += t2 - t1
+    plain = Simulator.from_scene(sc)
+    plain.simulate(16)
+    plain.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain.simulate(nsteps)
+    plain.synchronize()
+    tp = (time.perf_counter() - t0) / reps
+    adj.close()
+    sim.close()
+    plain.close()
+    tf, tbw = tf / reps, tbw / reps
+    return {"workload": "128x128 cloth over a 5,120-triangle icosphere, 200 steps, checkpoints every 20",
+            "forward_ms": 1e3 * tf, "backward_ms": 1e3 * tbw, "plain_ms": 1e3 * tp,
+            "forward_backward_vs_plain": (tf + tbw) / tp,
+            "gradient_rollouts_per_s": 1.0 / (tf + tbw)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +208,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB of cold memory "
                          "(the reference kernel for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -201,6 +289,11 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "k_verlet_large_traffic.txt"))
     if traffic:
         out["roofline"]["traffic"] = traffic
+    sim.close()
+    if not args.no_batch:
+        out["batched"] = batched_leg()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, budget_s=args.cpu_budget)
     if args.copy_probe_mib:
