@@ -23,6 +23,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
+WORKLOADS = {
+    "large": "large: 512x512 cloth, 262,144 vertices, 3,135,492 directed springs (structural/shear/bend), "
+             "12-capsule body of 501,120 triangles deformed and LBVH-refitted every frame, dt=5e-4",
+    "garment": "garment: 111x111 cloth, 12,321 vertices, 12-capsule body of 50,688 triangles deformed and "
+               "LBVH-rebuilt every frame, dt=1e-3",
+    "small": "small: 64x64 cloth, 4,096 vertices, static 5,120-triangle icosphere, dt=1e-3",
+}
 
 
 def algorithmic_bytes(sc):
@@ -71,12 +78,13 @@ def copy_probe(nbytes):
         hip.hipFree(ptr)
 
 
-def cpu_baseline(sc, budget_s=15.0, nthreads=1):
+def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
     """The oracle's FP32 path (the reference algorithm on the host) on a
-    bounded sample of the same workload: whole steps (deform + refit + step)
-    until ~budget_s of CPU time is spent."""
+    bounded sample of the same workload: whole steps (deform + refit or
+    rebuild + step; the step alone for a static body) until ~budget_s of CPU
+    time is spent."""
     import oracle as O
-    from simulator_amd import _lib as L  # noqa: F401  (scene arrays only)
+    from simulator_amd import _lib as L
     b = sc.body
     v4 = np.concatenate([b.verts, np.ones((b.verts.shape[0], 1), np.float32)], 1)
     rest = O.vertex_extend(v4, b.tris, b.tris, b.normals, sc.params.body_extend)[:, :3].copy()
@@ -90,7 +98,15 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1):
     m = b.motion
     steps = 0
     t0 = time.perf_counter()
+    moving = b.mode in (L.CC_BODY_REFIT, L.CC_BODY_REBUILD)
     while True:
+        if not moving:
+            x, xp, c = O.step_f32(x, xp, c, sc.springs, prm, pins, h, nthreads=nthreads)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or steps >= 200:
+                break
+            continue
         # frame motion (same formulas as the device deformation, host doubles)
         t = steps * float(np.float32(sc.params.dt))
         w = np.sin(2 * np.pi * m["freq"] * t)
@@ -102,20 +118,23 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1):
         breath = (1 + m["breath_amp"] * np.sin(2 * np.pi * m["breath_freq"] * t
                                                + caps["phase"].astype(np.float64))).astype(np.float32)
         verts = O.body_deform(rest, caps["vstart"], caps["p0"], caps["p1"], breath, R, T)
-        lb = O.prim_boxes(verts, d["leaf_tri"], sc.params.box_depth)
-        d["box_int"][:] = O.refit(d["childA"], d["childB"], d["parent_int"], lb)
-        d["box_leaf"][:] = lb
-        d["verts"] = np.ascontiguousarray(verts)
+        if b.mode == L.CC_BODY_REBUILD:
+            d = O.bvh_build(verts, b.tris, sc.params.box_depth)
+        else:
+            lb = O.prim_boxes(verts, d["leaf_tri"], sc.params.box_depth)
+            d["box_int"][:] = O.refit(d["childA"], d["childB"], d["parent_int"], lb)
+            d["box_leaf"][:] = lb
+            d["verts"] = np.ascontiguousarray(verts)
         h = O.BvhHandle(d)
         x, xp, c = O.step_f32(x, xp, c, sc.springs, prm, pins, h, nthreads=nthreads)
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 200:
             break
+    what = {L.CC_BODY_REFIT: "deform + refit + ", L.CC_BODY_REBUILD: "deform + rebuild + "}.get(b.mode, "")
     return {"value": sc.n * steps / el, "unit": "vertex-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{steps} whole steps (deform + refit + Verlet/spring/collision) of the same "
-                      f"512x512 / 501,120-triangle workload from the initial state, FP32 oracle, "
-                      f"{el:.1f} s"}
+            "sample": f"{steps} whole steps ({what}Verlet/spring/collision) of the same {label} workload "
+                      f"from the initial state, FP32 oracle, {el:.1f} s"}
 
 
 def batched_leg(warmup=16, steps=160, plain_steps=32):
@@ -208,6 +227,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["large", "garment", "small"], default="large",
+                    help="large (default, the headline): 512x512 over 501k refitted triangles; garment: "
+                         "111x111 over 50k triangles rebuilt every frame; small: 64x64 over a static sphere")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -222,7 +244,7 @@ def main():
     from simulator_amd import scene as S
     from simulator_amd.simulator import Simulator
 
-    sc = S.large()
+    sc = {"large": S.large, "garment": S.garment, "small": S.small}[args.workload]()
     sim = Simulator.from_scene(sc)
     # HIP events around every step graph and update-lane graph, on the
     # streams they run on, read while the following blocks run
@@ -271,9 +293,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded grid cloth and capsule-union body)",
-        "config": {"workload": "large: 512x512 cloth, 262,144 vertices, 3,135,492 directed springs "
-                               "(structural/shear/bend), 12-capsule body of 501,120 triangles deformed "
-                               "and LBVH-refitted every frame, dt=5e-4",
+        "config": {"workload": WORKLOADS[args.workload],
                    "vertices": sc.n, "springs_directed": ab["springs"],
                    "body_triangles": int(sc.body.tris.shape[0])},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -286,7 +306,7 @@ def main():
         "bvh_nodes_visited_per_s": visits / el_c,
         "bvh_nodes_per_query": visits / max(queries, 1),
     }
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "k_verlet_large_traffic.txt"))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "k_verlet_%s_traffic.txt" % args.workload))
     if traffic:
         out["roofline"]["traffic"] = traffic
     sim.close()
@@ -295,7 +315,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, budget_s=args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(sc, budget_s=args.cpu_budget, label=args.workload)
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib << 20)
     print(json.dumps(out))

@@ -31,7 +31,7 @@ constexpr int SB_WAVES = SB_THREADS / 64;
 constexpr int SB_ROWS = 8;          // rows of 64 elements per wave per sort tile
 constexpr int SB_TILE = SB_THREADS * SB_ROWS;
 constexpr int SB_BATCH = 4;         // elements per thread per round of the code and key loops
-constexpr uint32_t BB_MAX_M = 1u << 19;   // triangles per body (build levels: 512 -> 64 -> 8 -> 1 chunks)
+constexpr uint32_t BB_MAX_M = 1u << 19;   // triangles per batched body
 
 // Barrier ordering LDS accesses only: the global stores issued before it
 // stay in flight (a full __syncthreads waits for them).
@@ -500,6 +500,64 @@ __global__ __launch_bounds__(LV_C) void k_bb_level(const BodyDesc* __restrict__ 
 }
 
 // ------------------------------------------------------------ host side
+size_t level_chunks0(uint32_t m) { return (m + LV_C - 1) / LV_C; }
+
+std::vector<std::vector<LvBlock>> build_level_tables(const std::vector<BodyDesc>& h,
+                                                     const std::vector<uint32_t>& bodies) {
+  // level 0 over the leaves, level L+1 over LV_G chunks of level L, until a
+  // body's level has a single chunk; slot bases are per level, in list order
+  std::vector<std::vector<LvBlock>> out;
+  std::vector<uint32_t> chunks(h.size(), 0), prev_base(h.size(), 0), prev_n(h.size(), 0);
+  for (uint32_t j : bodies) chunks[j] = (uint32_t)level_chunks0(h[j].m);
+  for (int L = 0; L < BB_MAX_LEVELS; ++L) {
+    std::vector<LvBlock> t;
+    uint32_t base = 0;
+    std::vector<uint32_t> next(h.size(), 0), base_of(h.size(), 0);
+    for (uint32_t j : bodies) {
+      if (chunks[j] == 0) continue;
+      base_of[j] = base;
+      for (uint32_t q = 0; q < chunks[j]; ++q) {
+        LvBlock e{};
+        e.body = j;
+        e.q = q;
+        e.in_base = prev_base[j];
+        e.out_base = base;
+        e.n_in = L == 0 ? h[j].m : prev_n[j];
+        t.push_back(e);
+      }
+      base += chunks[j];
+      next[j] = chunks[j] > 1 ? (chunks[j] + LV_G - 1) / LV_G : 0;
+    }
+    if (t.empty()) break;
+    out.push_back(std::move(t));
+    prev_base = base_of;
+    prev_n = chunks;
+    chunks = next;
+  }
+  for (uint32_t j : bodies) CC_REQUIRE(chunks[j] == 0, "bottom-up build: too many levels");
+  return out;
+}
+
+void launch_build_levels(const BodyDesc* desc, const LvBlock* const* level, const uint32_t* n_level,
+                         const uint64_t* keys, const uint4* leaf_tri, const float* boxes, BItem* items0,
+                         BItem* items1, uint32_t* nitems0, uint32_t* nitems1, Node* nodes, int32_t* leaf_parent,
+                         float* root, float* leaf0, hipStream_t s) {
+  BItem* items[2] = {items0, items1};
+  uint32_t* nitems[2] = {nitems0, nitems1};
+  for (int L = 0; L < BB_MAX_LEVELS; ++L) {
+    if (!n_level[L]) continue;
+    const BItem* in = items[(L + 1) & 1];
+    const uint32_t* cin = nitems[(L + 1) & 1];
+    if (L == 0)
+      k_bb_level<true><<<n_level[L], LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, in, cin, items[0],
+                                                    nitems[0], nodes, leaf_parent, root, leaf0);
+    else
+      k_bb_level<false><<<n_level[L], LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, in, cin, items[L & 1],
+                                                     nitems[L & 1], nodes, leaf_parent, root, leaf0);
+    CC_HIP(hipGetLastError());
+  }
+}
+
 BatchBodies::~BatchBodies() {
   for (int q = 0; q < 2; ++q) {
     if (ev_motion[q]) {
@@ -641,7 +699,7 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return h[a].m > h[b].m; });
   size_t chunks0 = 0;
-  for (uint32_t j = 0; j < nb; ++j) chunks0 += (h[j].m + LV_C - 1) / LV_C;
+  for (uint32_t j = 0; j < nb; ++j) chunks0 += level_chunks0(h[j].m);
   for (auto& b : items) b.alloc(chunks0 * LV_CAP);
   for (auto& b : nitems) {
     b.alloc(chunks0);
@@ -663,38 +721,11 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
     upload_list(P.deform, P.n_deform, tdef, s);
     upload_list(P.boxes, P.n_boxes, tbox, s);
     upload_list(P.sort, P.n_sort, lsort, s);
-    // build levels: level 0 over the leaves, level L+1 over LV_G chunks of
-    // level L, until a body's level has a single chunk
-    std::vector<uint32_t> chunks(nb, 0), prev_base(nb, 0), prev_n(nb, 0);
-    for (uint32_t j : lbuild) chunks[j] = (h[j].m + LV_C - 1) / LV_C;
+    const std::vector<std::vector<LvBlock>> t = build_level_tables(h, lbuild);
     for (int L = 0; L < BB_MAX_LEVELS; ++L) {
-      std::vector<LvBlock> t;
-      uint32_t base = 0;
-      std::vector<uint32_t> next_chunks(nb, 0), base_of(nb, 0);
-      for (uint32_t j : lbuild) {
-        if (chunks[j] == 0) continue;
-        base_of[j] = base;
-        const uint32_t n_in = L == 0 ? h[j].m : prev_n[j];
-        for (uint32_t q = 0; q < chunks[j]; ++q) {
-          LvBlock e{};
-          e.body = j;
-          e.q = q;
-          e.in_base = prev_base[j];
-          e.out_base = base;
-          e.n_in = n_in;
-          t.push_back(e);
-        }
-        base += chunks[j];
-        next_chunks[j] = chunks[j] > 1 ? (chunks[j] + LV_G - 1) / LV_G : 0;
-      }
-      P.n_level[L] = (uint32_t)t.size();
-      P.level[L].upload(t.data(), std::max<size_t>(t.size(), 1), s);
-      if (t.empty()) P.level[L].zero(s);
-      prev_base = base_of;
-      prev_n = chunks;
-      chunks = next_chunks;
+      P.n_level[L] = L < (int)t.size() ? (uint32_t)t[L].size() : 0u;
+      if (P.n_level[L]) P.level[L].upload(t[L].data(), t[L].size(), s);
     }
-    for (uint32_t j : lbuild) CC_REQUIRE(chunks[j] == 0, "batch body: too many build levels");
   }
   // frame 0 for every body
   upload_motion(0, 1, s);
@@ -738,19 +769,10 @@ void BatchBodies::launch(const BodyPass& P, int k, hipStream_t s) {
   if (P.n_sort)
     k_bb_sort<<<P.n_sort, SB_THREADS, 0, s>>>(desc.p, P.sort.p, boxes.p, partial.p, kv[0].p, kv[1].p, keys.p,
                                                leaf_tri.p, tris.p, root.p);
-  for (int L = 0; L < BB_MAX_LEVELS; ++L) {
-    if (!P.n_level[L]) continue;
-    const BItem* in = items[(L + 1) & 1].p;
-    const uint32_t* cin = nitems[(L + 1) & 1].p;
-    if (L == 0)
-      k_bb_level<true><<<P.n_level[L], LV_C, 0, s>>>(desc.p, P.level[L].p, keys.p, leaf_tri.p, boxes.p, in, cin,
-                                                      items[0].p, nitems[0].p, nodes.p, leaf_parent.p, root.p,
-                                                      leaf0.p);
-    else
-      k_bb_level<false><<<P.n_level[L], LV_C, 0, s>>>(desc.p, P.level[L].p, keys.p, leaf_tri.p, boxes.p, in, cin,
-                                                       items[L & 1].p, nitems[L & 1].p, nodes.p, leaf_parent.p,
-                                                       root.p, leaf0.p);
-  }
+  const LvBlock* tabs[BB_MAX_LEVELS];
+  for (int L = 0; L < BB_MAX_LEVELS; ++L) tabs[L] = P.level[L].p;
+  launch_build_levels(desc.p, tabs, P.n_level, keys.p, leaf_tri.p, boxes.p, items[0].p, items[1].p, nitems[0].p,
+                      nitems[1].p, nodes.p, leaf_parent.p, root.p, leaf0.p, s);
   CC_HIP(hipGetLastError());
 }
 

@@ -12,35 +12,6 @@ namespace cc {
 constexpr int BB_MF = 12 + MAX_CAPS;   // floats per motion frame and body
 constexpr int BB_FRAMES = 16;          // motion frames held on the device (one step graph)
 
-// One body of the batch; offsets index the concatenated arrays.
-struct BodyDesc {
-  uint32_t nv, m, ncaps, mode;
-  uint32_t v0;    // first vertex
-  uint32_t t0;    // first triangle == first leaf / key
-  uint32_t n0;    // first internal node
-  uint32_t c0;    // first capsule (p0 / p1 entries)
-  uint32_t cv0;   // first capsule vertex-range entry (ncaps + 1 per body)
-  uint32_t p0;    // first partial root box (one per 256 triangles)
-  float depth;
-  uint32_t pad;
-};
-
-// A node finished by one level of the bottom-up build and handed to the
-// next (its sorted-leaf range, Karras index and box).
-struct alignas(16) BItem {
-  int32_t l, r, idx, pad;
-  float box[6];
-  float pad2[2];
-};
-
-// One workgroup of a build level: chunk q of the body's level, input slots
-// from in_base (n_in of them; leaves at level 0), output slots at out_base.
-struct LvBlock {
-  uint32_t body, q, in_base, out_base, n_in, pad[3];
-};
-
-constexpr int BB_MAX_LEVELS = 5;       // 2^19 leaves: 512 -> 64 -> 8 -> 1 chunks
-
 // Work lists of one kind of pass: block tables {body, first, count, aux} of
 // the grid-wide kernels, the body list of the per-body sort, and the
 // workgroups of each build level.

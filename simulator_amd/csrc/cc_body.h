@@ -8,6 +8,8 @@
 // own build scratch, so several frames can be updated at the same time.
 #pragma once
 
+#include <vector>
+
 #include "cc_internal.h"
 
 namespace cc {
@@ -34,6 +36,49 @@ struct MotionFrame {
 // precision with one rounding to float.
 MotionFrame motion_of(const MotionParams& a, const float* phase_host, uint64_t frame);
 
+// One body of the batch; offsets index the concatenated arrays.
+struct BodyDesc {
+  uint32_t nv, m, ncaps, mode;
+  uint32_t v0;    // first vertex
+  uint32_t t0;    // first triangle == first leaf / key
+  uint32_t n0;    // first internal node
+  uint32_t c0;    // first capsule (p0 / p1 entries)
+  uint32_t cv0;   // first capsule vertex-range entry (ncaps + 1 per body)
+  uint32_t p0;    // first partial root box (one per 256 triangles)
+  float depth;
+  uint32_t pad;
+};
+
+// A node finished by one level of the bottom-up build and handed to the
+// next (its sorted-leaf range, Karras index and box).
+struct alignas(16) BItem {
+  int32_t l, r, idx, pad;
+  float box[6];
+  float pad2[2];
+};
+
+// One workgroup of a build level: chunk q of the body's level, input slots
+// from in_base (n_in of them; leaves at level 0), output slots at out_base.
+struct LvBlock {
+  uint32_t body, q, in_base, out_base, n_in, pad[3];
+};
+
+constexpr int BB_MAX_LEVELS = 8;       // 2^30 leaves: 2^20 -> 2^17 -> ... -> 4 -> 1 chunks
+constexpr uint32_t LV_MAX_LEAVES = 1u << 21;   // leaf indices of the bottom-up build carry 21 bits
+
+// The bottom-up build (bbody.hip): work tables of its levels for the given
+// bodies (level 0: chunks of 1024 leaves, level L+1: 8 chunks of level L),
+// the number of level-0 chunks of m leaves, and the launches of all levels.
+// keys / leaf_tri / boxes / leaf_parent / nodes are indexed through each
+// body's t0 / n0; root and leaf0 hold 6 floats per body.
+std::vector<std::vector<LvBlock>> build_level_tables(const std::vector<BodyDesc>& h,
+                                                     const std::vector<uint32_t>& bodies);
+size_t level_chunks0(uint32_t m);
+void launch_build_levels(const BodyDesc* desc, const LvBlock* const* level, const uint32_t* n_level,
+                         const uint64_t* keys, const uint4* leaf_tri, const float* boxes, BItem* items0,
+                         BItem* items1, uint32_t* nitems0, uint32_t* nitems1, Node* nodes, int32_t* leaf_parent,
+                         float* root, float* leaf0, hipStream_t s);
+
 // One frame of the body as the collision query reads it.
 struct Slot {
   DBuf<float> verts;       // [3*nv] pose
@@ -53,6 +98,7 @@ struct Lane {
   DBuf<float> partial;              // [6*nblk_box] per-block root boxes
   DBuf<uint32_t> code[2], idx[2];   // radix sort ping-pong
   DBuf<uint32_t> hist;              // [256 * sort blocks]
+  DBuf<uint32_t> digit_total;       // [256] keys per digit of the current pass
   DBuf<uint64_t> keys;              // [m] sorted morton << 32 | primitive
   DBuf<int32_t> leaf_parent;        // [m]
   // refit description: internal nodes whose leaf range crosses a chunk
@@ -70,8 +116,11 @@ struct Lane {
   // written by every refit
   DBuf<float> top_child;            // [12 * max(m-1, 1)] boxes of the entries' non-top children
   DBuf<float> top_scratch;          // set-up work space and the top pass beyond its LDS size
+  // bottom-up build (rebuilt bodies): unfinished nodes between levels
+  DBuf<BItem> items[2];
+  DBuf<uint32_t> nitems[2];
 
-  void alloc(uint32_t m, bool full_lane, hipStream_t s);
+  void alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s);
 };
 
 // Host copy of a built tree in the reference encoding (see cc_bvh_export):
@@ -90,6 +139,14 @@ struct Body {
   DBuf<float> rest;     // [3*nv] rest pose (after vertex_extend)
   DBuf<uint32_t> tris;  // [3*m]
   DBuf<uint4> leaf_tri; // [m] sorted leaves of the shared topology (static and refitted bodies)
+
+  // A rebuilt body of 2..2^21 triangles builds its tree and boxes bottom-up
+  // (the batched build's level kernels) instead of the Karras searches, the
+  // refit description and the refit.
+  bool bottom_up = false;
+  DBuf<BodyDesc> bu_desc;           // [1] offsets 0
+  DBuf<LvBlock> bu_level[BB_MAX_LEVELS];
+  uint32_t bu_n[BB_MAX_LEVELS] = {};
 
   // capsule decomposition for the deformation
   uint32_t ncaps = 0;
@@ -129,6 +186,8 @@ struct Body {
   // launches, asynchronous on s
   void deform(int k, uint64_t frame, hipStream_t s);   // rest -> slot k pose of `frame`
   void build(int k, int l, hipStream_t s);      // full LBVH build of slot k on lane l (includes the refit)
+  void build_levels(int k, int l, hipStream_t s);   // bottom-up tree + boxes from lane l's keys and boxes
+  void prim_boxes_of(int k, int l, hipStream_t s);  // lane l's expanded boxes of slot k's pose
   void top_setup(int k, int l, hipStream_t s);  // refit description of slot k's topology into lane l
   void refit(int k, int l, hipStream_t s);      // boxes of slot k
   void update(int k, int l, uint64_t frame, hipStream_t s);   // per-mode update of slot k to `frame`

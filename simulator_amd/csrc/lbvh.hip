@@ -158,27 +158,25 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
   hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of len counters by one 1024-thread block
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint32_t len) {
-  __shared__ uint32_t sums[1024];
-  const uint32_t per = (len + 1023) / 1024;
-  const uint32_t b = threadIdx.x * per, e = min(len, b + per);
-  uint32_t s = 0;
-  for (uint32_t i = b; i < e; ++i) s += data[i];
-  sums[threadIdx.x] = s;
-  __syncthreads();
-  for (uint32_t off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
-    __syncthreads();
-    sums[threadIdx.x] += v;
-    __syncthreads();
+// Per digit, the exclusive prefix over blocks of the block counts (in place)
+// and the digit's total: one wave per digit, 64 blocks per round.
+__global__ __launch_bounds__(256) void k_radix_rowscan(uint32_t* __restrict__ hist, uint32_t nblk,
+                                                       uint32_t* __restrict__ total) {
+  const uint32_t lane = threadIdx.x & 63u, d = blockIdx.x * 4 + (threadIdx.x >> 6);
+  uint32_t* row = hist + (size_t)d * nblk;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 64) {
+    const uint32_t b = b0 + lane;
+    const uint32_t v = b < nblk ? row[b] : 0u;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, off, 64);
+      if (lane >= (uint32_t)off) incl += u;
+    }
+    if (b < nblk) row[b] = carry + incl - v;
+    carry += __shfl(incl, 63, 64);
   }
-  uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
-  for (uint32_t i = b; i < e; ++i) {
-    const uint32_t v = data[i];
-    data[i] = run;
-    run += v;
-  }
+  if (lane == 0) total[d] = carry;
 }
 
 // Scatter with stable ranks: within a wave, lanes holding the same digit are
@@ -186,13 +184,27 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin, uint32_t n,
                                                                 int shift, const uint32_t* __restrict__ hist,
+                                                                const uint32_t* __restrict__ total,
                                                                 uint32_t nblk, uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout) {
   __shared__ uint32_t goff[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[SORT_THREADS / 64][256];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  goff[tid] = hist[(size_t)tid * nblk + blockIdx.x];
+  {  // digit bases: exclusive scan of the digit totals (one wave per 64 digits)
+    const uint32_t v = total[tid];
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, off, 64);
+      if (lane >= (uint32_t)off) incl += u;
+    }
+    run[tid] = incl;                    // wave-inclusive; combined below
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t q = 0; q < w; ++q) base += run[q * 64 + 63];
+    goff[tid] = base + incl - v + hist[(size_t)tid * nblk + blockIdx.x];
+    __syncthreads();
+  }
   run[tid] = 0;
   const uint32_t base = blockIdx.x * SORT_TILE;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -254,7 +266,22 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ key
 }
 
 // ------------------------------------------------------------ host side
-void Lane::alloc(uint32_t m, bool full_lane, hipStream_t s) {
+void Body::prim_boxes_of(int k, int l, hipStream_t s) {
+  Lane& L = lane[l];
+  k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(slot[k].verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
+  CC_HIP(hipGetLastError());
+}
+
+void Body::build_levels(int k, int l, hipStream_t s) {
+  Lane& L = lane[l];
+  Slot& S = slot[k];
+  const LvBlock* tabs[BB_MAX_LEVELS];
+  for (int q = 0; q < BB_MAX_LEVELS; ++q) tabs[q] = bu_level[q].p;
+  launch_build_levels(bu_desc.p, tabs, bu_n, L.keys.p, leaf_tri_of(k), L.boxes.p, L.items[0].p, L.items[1].p,
+                      L.nitems[0].p, L.nitems[1].p, S.nodes.p, L.leaf_parent.p, S.root.p, S.leaf0_box.p, s);
+}
+
+void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
   full = full_lane;
   const size_t mi = std::max<uint32_t>(m, 2) - 1;
   top_child.alloc(12 * mi);
@@ -268,6 +295,15 @@ void Lane::alloc(uint32_t m, bool full_lane, hipStream_t s) {
     idx[b].alloc(m);
   }
   hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
+  digit_total.alloc(256);
+  if (bottom_up) {
+    const size_t c0 = level_chunks0(m);
+    for (int q = 0; q < 2; ++q) {
+      items[q].alloc(c0 * 128);
+      nitems[q].alloc(c0);
+      nitems[q].zero(s);
+    }
+  }
   keys.alloc(m);
   leaf_parent.alloc(m);
   leaf_parent.zero(s);
@@ -333,7 +369,21 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
     S.motion.alloc(12 + (size_t)ncaps);
     S.motion.zero(s);
   }
-  for (int l = 0; l < nlanes; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, s);
+  bottom_up = mode == CC_BODY_REBUILD && m >= 2 && m <= LV_MAX_LEAVES;
+  if (bottom_up) {
+    BodyDesc d{};
+    d.nv = nv;
+    d.m = m;
+    d.mode = mode;
+    d.depth = depth;
+    bu_desc.upload(&d, 1, s);
+    const std::vector<std::vector<LvBlock>> t = build_level_tables(std::vector<BodyDesc>{d}, {0u});
+    for (int q = 0; q < BB_MAX_LEVELS; ++q) {
+      bu_n[q] = q < (int)t.size() ? (uint32_t)t[q].size() : 0u;
+      if (bu_n[q]) bu_level[q].upload(t[q].data(), t[q].size(), s);
+    }
+  }
+  for (int l = 0; l < nlanes; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, bottom_up, s);
 }
 
 void Body::deform(int k, uint64_t frame, hipStream_t s) {
@@ -358,15 +408,19 @@ void Body::build(int k, int l, hipStream_t s) {
   int src = 0;
   for (int shift = 0; shift < 32; shift += 8) {
     k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, m, shift, L.hist.p, nblk);
-    k_scan<<<1, 1024, 0, s>>>(L.hist.p, 256 * nblk);
-    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p, nblk,
-                                                  L.code[src ^ 1].p, L.idx[src ^ 1].p);
+    k_radix_rowscan<<<64, 256, 0, s>>>(L.hist.p, nblk, L.digit_total.p);
+    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p,
+                                                  L.digit_total.p, nblk, L.code[src ^ 1].p, L.idx[src ^ 1].p);
     CC_HIP(hipGetLastError());
     src ^= 1;
   }
   k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(L.code[src].p, L.idx[src].p, tris.p, m, L.keys.p,
                                                  leaf_tri_of(k));
   CC_HIP(hipGetLastError());
+  if (bottom_up) {                      // tree, parent links and boxes, bottom-up
+    build_levels(k, l, s);
+    return;
+  }
   if (m >= 2) {
     k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(L.keys.p, (int32_t)m, S.nodes.p, L.leaf_parent.p,
                                                   L.top_count.p);

@@ -481,6 +481,11 @@ void Body::top_setup(int k, int l, hipStream_t s) {
 
 void Body::refit(int k, int l, hipStream_t s) {
   if (mode == CC_BODY_NONE || m == 0) return;
+  if (bottom_up) {                      // same keys, so the same tree; new boxes
+    prim_boxes_of(k, l, s);
+    build_levels(k, l, s);
+    return;
+  }
   Lane& L = lane[l];
   Lane& T = topo(l);
   Slot& S = slot[k];
