@@ -20,11 +20,6 @@
 
 namespace cc {
 
-struct InstDesc {
-  StepArgs a;
-  BvhView bvh;
-};
-
 template <int TILE>
 __global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__ sin_, float* __restrict__ sout,
                                                        float* __restrict__ contact, const uint32_t* __restrict__ row,
@@ -54,12 +49,14 @@ struct Batch {
   DBuf<uint2> rec;
   DBuf<int32_t> spill;
   BatchBodies bodies;
-  DBuf<InstDesc> insts;
+  DBuf<InstDesc> insts[2];                   // per body slot (frame g uses slot g % 2 == cur)
   uint64_t frame = 0;
   // the chosen launch shape
   int tile = 256, order = 0;
   DBuf<uint4> blocks;
   uint32_t nblocks = 0;
+  DBuf<uint4> blocks256;                     // 256-vertex tiles in the chosen order (fused with the sort)
+  uint32_t nblocks256 = 0;
   int cur = 0;
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
 
@@ -89,11 +86,14 @@ struct Batch {
     const float* in = state[cur].p;
     float* out = state[cur ^ 1].p;
     if (T == 64)
-      k_verlet_batch<64><<<nblk, 64, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+      k_verlet_batch<64><<<nblk, 64, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
+                                             spill.p);
     else if (T == 128)
-      k_verlet_batch<128><<<nblk, 128, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+      k_verlet_batch<128><<<nblk, 128, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
+                                               spill.p);
     else
-      k_verlet_batch<256><<<nblk, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, insts.p, blk, ntotal, spill.p);
+      k_verlet_batch<256><<<nblk, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
+                                               spill.p);
     CC_HIP(hipGetLastError());
   }
 
@@ -142,7 +142,31 @@ struct Batch {
     const std::vector<uint4> t = block_table(tile, order);
     blocks.upload(t.data(), t.size(), stream);
     nblocks = (uint32_t)t.size();
+    const std::vector<uint4> t256 = block_table(256, order);
+    blocks256.upload(t256.data(), t256.size(), stream);
+    nblocks256 = (uint32_t)t256.size();
     CC_HIP(hipStreamSynchronize(stream));
+  }
+
+  // One step of frame `frame` (cur == frame % 2): the moving bodies of frame
+  // + 1 are posed, sorted and built into the other slot while the cloths step
+  // against the bodies of this frame; k selects frame + 1's motion.
+  void one_step(int k, hipStream_t s) {
+    const int nxt = cur ^ 1;
+    if (bodies.moving) {
+      bodies.pre(bodies.step, k, nxt, s);
+      if (bodies.step.n_sort) {
+        StepLaunch st{state[cur].p, state[cur ^ 1].p, contact.p, row.p, rec.p, insts[cur].p,
+                      blocks256.p, nblocks256, ntotal, spill.p};
+        bodies.sort_with_step(bodies.step, nxt, st, s);
+      } else {
+        launch(tile, blocks.p, nblocks, s);
+      }
+      bodies.post(bodies.step, nxt, s);
+    } else {
+      launch(tile, blocks.p, nblocks, s);
+    }
+    cur ^= 1;
   }
 
   hipGraphExec_t capture() {
@@ -150,11 +174,7 @@ struct Batch {
     hipGraph_t graph = nullptr;
     CC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < GRAPH_STEPS_B; ++k) {
-        bodies.launch(bodies.step, k, stream);
-        launch(tile, blocks.p, nblocks, stream);
-        cur ^= 1;
-      }
+      for (int k = 0; k < GRAPH_STEPS_B; ++k) one_step(k, stream);
     } catch (...) {
       cur = cur0;
       hipGraph_t junk = nullptr;
@@ -171,25 +191,21 @@ struct Batch {
     return g;
   }
 
-  // Step g: the moving bodies are posed at frame g and rebuilt / refitted,
-  // then every cloth steps against its body (the single simulator's order).
+  // Step g collides every cloth with its body's pose of frame g (the single
+  // simulator's order); that pose was built during step g - 1 (or at set-up).
   void step(int nsteps, hipStream_t s) {
     if (!s) s = stream;
     while (nsteps > 0) {
       if (nsteps >= GRAPH_STEPS_B) {
         hipGraphExec_t& g = graphs[cur];
         if (!g) g = capture();
-        bodies.upload_motion(frame, GRAPH_STEPS_B, s);
+        bodies.upload_motion(frame + 1, GRAPH_STEPS_B, s);
         CC_HIP(hipGraphLaunch(g, s));
         nsteps -= GRAPH_STEPS_B;             // even: the parity is unchanged
         frame += GRAPH_STEPS_B;
       } else {
-        if (bodies.moving) {
-          bodies.upload_motion(frame, 1, s);
-          bodies.launch(bodies.step, 0, s);
-        }
-        launch(tile, blocks.p, nblocks, s);
-        cur ^= 1;
+        bodies.upload_motion(frame + 1, 1, s);
+        one_step(0, s);
         --nsteps;
         ++frame;
       }
@@ -272,8 +288,10 @@ int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, con
       desc[k].a = step_args(params[k], cloths[k].ntypes);
       desc[k].bvh = BvhView{nullptr, nullptr, nullptr, nullptr, 0};
     }
-    for (uint32_t j = 0; j < B.bodies.nb; ++j) desc[B.bodies.inst[j]].bvh = B.bodies.view(j);
-    B.insts.upload(desc.data(), desc.size(), s);
+    for (int slot = 0; slot < 2; ++slot) {
+      for (uint32_t j = 0; j < B.bodies.nb; ++j) desc[B.bodies.inst[j]].bvh = B.bodies.view(j, slot);
+      B.insts[slot].upload(desc.data(), desc.size(), s);
+    }
     CC_HIP(hipStreamSynchronize(s));
     B.autotune();
     *out = h.release();
@@ -352,12 +370,8 @@ int cc_batch_body_export(cc_batch* b, uint32_t inst, float* verts3, uint64_t* ke
     while (j < Y.nb && Y.inst[j] != inst) ++j;
     CC_REQUIRE(j < Y.nb, "cc_batch_body_export: instance has no body");
     CC_HIP(hipDeviceSynchronize());
-    if (Y.moving) {                 // the pose of the current frame (the next step poses it again)
-      Y.upload_motion(B.frame, 1, B.stream);
-      Y.launch(Y.step, 0, B.stream);
-    }
-    Y.export_tree(j, verts3, keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
-                  B.stream);
+    Y.export_tree(j, B.cur, verts3, keys, order, childA, childB, parent_int, parent_leaf, box_int, box_leaf, root6,
+                  B.stream);            // slot cur holds the current frame
   });
 }
 

@@ -21,7 +21,7 @@
 
 #include "cc_bbody.h"
 #include "cc_lbvh.h"
-#include "cc_sim.h"
+#include "cc_step.h"
 
 namespace cc {
 
@@ -110,20 +110,16 @@ __global__ __launch_bounds__(BB_BLOCK) void k_bb_boxes(const BodyDesc* __restric
 // rows in LDS, so a tile needs three barriers whatever its size (LDS-only
 // ones: a pass's scattered stores are waited for once, at its end).
 // Element order = (wave, row, lane) order, so the pass is stable.
-__global__ __launch_bounds__(SB_THREADS) void k_bb_sort(const BodyDesc* __restrict__ desc,
-                                                        const uint32_t* __restrict__ list,
-                                                        const float* __restrict__ boxes,
-                                                        const float* __restrict__ partial, uint2* kv0, uint2* kv1,
-                                                        uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri,
-                                                        const uint32_t* __restrict__ tris,
-                                                        float* __restrict__ root) {
+__device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, const float* __restrict__ boxes,
+                               const float* __restrict__ partial, uint2* kv0, uint2* kv1,
+                               uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri,
+                               const uint32_t* __restrict__ tris, float* __restrict__ root) {
   __shared__ uint32_t hist[4][256];
   __shared__ uint32_t wcnt[SB_WAVES][256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wtot[SB_WAVES];
   __shared__ float red[SB_WAVES][6];
   __shared__ float sroot[6];
-  const uint32_t j = list[blockIdx.x];
   const BodyDesc d = desc[j];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t m = d.m, t0 = d.t0;
@@ -265,6 +261,43 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort(const BodyDesc* __restri
       leaf_tri[t0 + i] = make_uint4(tv[u][0], tv[u][1], tv[u][2], e[u].y);
     }
   }
+}
+
+__global__ __launch_bounds__(SB_THREADS) void k_bb_sort(const BodyDesc* __restrict__ desc,
+                                                        const uint32_t* __restrict__ list,
+                                                        const float* __restrict__ boxes,
+                                                        const float* __restrict__ partial, uint2* kv0, uint2* kv1,
+                                                        uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri,
+                                                        const uint32_t* __restrict__ tris,
+                                                        float* __restrict__ root) {
+  bb_sort(list[blockIdx.x], desc, boxes, partial, kv0, kv1, keys, leaf_tri, tris, root);
+}
+
+// The sort's workgroups, then one cloth step of the batch in blocks of four
+// 256-vertex tiles (the sort of frame g + 1 and the step of frame g read and
+// write disjoint buffers).
+__global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __restrict__ desc,
+                                                             const uint32_t* __restrict__ list, uint32_t nsort,
+                                                             const float* __restrict__ boxes,
+                                                             const float* __restrict__ partial, uint2* kv0,
+                                                             uint2* kv1, uint64_t* __restrict__ keys,
+                                                             uint4* __restrict__ leaf_tri,
+                                                             const uint32_t* __restrict__ tris,
+                                                             float* __restrict__ root, StepLaunch st) {
+  if (blockIdx.x < nsort) {
+    bb_sort(list[blockIdx.x], desc, boxes, partial, kv0, kv1, keys, leaf_tri, tris, root);
+    return;
+  }
+  const uint32_t e = (blockIdx.x - nsort) * (SB_THREADS / 256) + (threadIdx.x >> 8);
+  if (e >= st.nblocks) return;
+  const uint4 b = st.blocks[e];
+  const uint32_t lt = threadIdx.x & 255u;
+  if (lt >= b.z) return;
+  const InstDesc dd = st.insts[b.x];
+  const uint32_t i = b.y + lt;
+  uint32_t visits = 0;
+  step_vertex<false>(i, st.in, st.out, st.contact, st.row, st.rec, dd.a, dd.bvh, Spill{st.spill + i, st.ntotal},
+                     visits, nullptr, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------ tree + boxes
@@ -669,7 +702,7 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
   if (nb == 0) return;
   desc.upload(h.data(), nb, s);
   rest.upload(h_rest.data(), h_rest.size(), s);
-  verts.upload(h_rest.data(), h_rest.size(), s);
+  for (auto& b : verts) b.upload(h_rest.data(), h_rest.size(), s);
   tris.upload(h_tris.data(), h_tris.size(), s);
   capv.upload(h_capv.data(), std::max<size_t>(h_capv.size(), 1), s);
   capp0.upload(h_p0.data(), std::max<size_t>(h_p0.size(), 1), s);
@@ -678,15 +711,17 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
   partial.alloc(6 * (size_t)p_sum);
   for (auto& b : kv) b.alloc(m_sum);
   keys.alloc(m_sum);
-  leaf_tri.alloc(m_sum);
-  leaf_parent.alloc(m_sum);
-  leaf_parent.zero(s);
-  nodes.alloc(n_sum);
-  nodes.zero(s);
-  root.alloc(6 * (size_t)nb);
-  root.zero(s);
-  leaf0.alloc(6 * (size_t)nb);
-  leaf0.zero(s);
+  for (int q = 0; q < 2; ++q) {
+    leaf_tri[q].alloc(m_sum);
+    leaf_parent[q].alloc(m_sum);
+    leaf_parent[q].zero(s);
+    nodes[q].alloc(n_sum);
+    nodes[q].zero(s);
+    root[q].alloc(6 * (size_t)nb);
+    root[q].zero(s);
+    leaf0[q].alloc(6 * (size_t)nb);
+    leaf0[q].zero(s);
+  }
   motion.alloc((size_t)BB_FRAMES * nb * BB_MF);
   motion.zero(s);
   for (int q = 0; q < 2; ++q) {
@@ -727,18 +762,19 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
       if (P.n_level[L]) P.level[L].upload(t[L].data(), t[L].size(), s);
     }
   }
-  // frame 0 for every body
+  // frame 0 for every body, in both slots
   upload_motion(0, 1, s);
-  launch(all, 0, s);
+  launch(all, 0, 0, s);
+  launch(all, 0, 1, s);
 }
 
-BvhView BatchBodies::view(uint32_t j) const {
+BvhView BatchBodies::view(uint32_t j, int slot) const {
   const BodyDesc& d = h[j];
   BvhView v;
-  v.nodes = nodes.p + d.n0;
-  v.leaf_tri = leaf_tri.p + d.t0;
-  v.verts = verts.p + 3 * (size_t)d.v0;
-  v.leaf0_box = leaf0.p + 6 * (size_t)j;
+  v.nodes = nodes[slot].p + d.n0;
+  v.leaf_tri = leaf_tri[slot].p + d.t0;
+  v.verts = verts[slot].p + 3 * (size_t)d.v0;
+  v.leaf0_box = leaf0[slot].p + 6 * (size_t)j;
   v.m = d.m;
   return v;
 }
@@ -759,34 +795,50 @@ void BatchBodies::upload_motion(uint64_t frame0, int nframes, hipStream_t s) {
   hm ^= 1;
 }
 
-void BatchBodies::launch(const BodyPass& P, int k, hipStream_t s) {
+void BatchBodies::pre(const BodyPass& P, int k, int slot, hipStream_t s) {
   if (nb == 0) return;
   if (P.n_deform)
-    k_bb_deform<<<P.n_deform, BB_BLOCK, 0, s>>>(desc.p, P.deform.p, rest.p, verts.p, capv.p, capp0.p, capp1.p,
-                                                 motion.p, nb, k);
+    k_bb_deform<<<P.n_deform, BB_BLOCK, 0, s>>>(desc.p, P.deform.p, rest.p, verts[slot].p, capv.p, capp0.p,
+                                                 capp1.p, motion.p, nb, k);
   if (P.n_boxes)
-    k_bb_boxes<<<P.n_boxes, BB_BLOCK, 0, s>>>(desc.p, P.boxes.p, verts.p, tris.p, boxes.p, partial.p);
-  if (P.n_sort)
-    k_bb_sort<<<P.n_sort, SB_THREADS, 0, s>>>(desc.p, P.sort.p, boxes.p, partial.p, kv[0].p, kv[1].p, keys.p,
-                                               leaf_tri.p, tris.p, root.p);
-  const LvBlock* tabs[BB_MAX_LEVELS];
-  for (int L = 0; L < BB_MAX_LEVELS; ++L) tabs[L] = P.level[L].p;
-  launch_build_levels(desc.p, tabs, P.n_level, keys.p, leaf_tri.p, boxes.p, items[0].p, items[1].p, nitems[0].p,
-                      nitems[1].p, nodes.p, leaf_parent.p, root.p, leaf0.p, s);
+    k_bb_boxes<<<P.n_boxes, BB_BLOCK, 0, s>>>(desc.p, P.boxes.p, verts[slot].p, tris.p, boxes.p, partial.p);
   CC_HIP(hipGetLastError());
 }
 
-void BatchBodies::export_tree(uint32_t j, float* verts3, uint64_t* okeys, uint32_t* order, int32_t* childA,
-                              int32_t* childB, int32_t* parent_int, int32_t* parent_leaf, float* box_int,
-                              float* box_leaf, float* root6, hipStream_t s) {
+void BatchBodies::sort(const BodyPass& P, int slot, hipStream_t s) {
+  if (nb == 0 || !P.n_sort) return;
+  k_bb_sort<<<P.n_sort, SB_THREADS, 0, s>>>(desc.p, P.sort.p, boxes.p, partial.p, kv[0].p, kv[1].p, keys.p,
+                                             leaf_tri[slot].p, tris.p, root[slot].p);
+  CC_HIP(hipGetLastError());
+}
+
+void BatchBodies::sort_with_step(const BodyPass& P, int slot, const StepLaunch& st, hipStream_t s) {
+  const uint32_t nstep = (st.nblocks + SB_THREADS / 256 - 1) / (SB_THREADS / 256);
+  k_bb_sort_step<<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p, kv[0].p,
+                                                          kv[1].p, keys.p, leaf_tri[slot].p, tris.p, root[slot].p,
+                                                          st);
+  CC_HIP(hipGetLastError());
+}
+
+void BatchBodies::post(const BodyPass& P, int slot, hipStream_t s) {
+  if (nb == 0) return;
+  const LvBlock* tabs[BB_MAX_LEVELS];
+  for (int L = 0; L < BB_MAX_LEVELS; ++L) tabs[L] = P.level[L].p;
+  launch_build_levels(desc.p, tabs, P.n_level, keys.p, leaf_tri[slot].p, boxes.p, items[0].p, items[1].p,
+                      nitems[0].p, nitems[1].p, nodes[slot].p, leaf_parent[slot].p, root[slot].p, leaf0[slot].p, s);
+}
+
+void BatchBodies::export_tree(uint32_t j, int slot, float* verts3, uint64_t* okeys, uint32_t* order,
+                              int32_t* childA, int32_t* childB, int32_t* parent_int, int32_t* parent_leaf,
+                              float* box_int, float* box_leaf, float* root6, hipStream_t s) {
   CC_REQUIRE(j < nb, "batch body: no such body");
   const BodyDesc& d = h[j];
   if (verts3)
-    CC_HIP(hipMemcpyAsync(verts3, verts.p + 3 * (size_t)d.v0, 3 * (size_t)d.nv * sizeof(float),
+    CC_HIP(hipMemcpyAsync(verts3, verts[slot].p + 3 * (size_t)d.v0, 3 * (size_t)d.nv * sizeof(float),
                           hipMemcpyDeviceToHost, s));
-  export_tree_raw(d.m, keys.p + d.t0, leaf_tri.p + d.t0, leaf_parent.p + d.t0, nodes.p + d.n0,
-                  root.p + 6 * (size_t)j, leaf0.p + 6 * (size_t)j, okeys, order, childA, childB, parent_int,
-                  parent_leaf, box_int, box_leaf, root6, s);
+  export_tree_raw(d.m, keys.p + d.t0, leaf_tri[slot].p + d.t0, leaf_parent[slot].p + d.t0, nodes[slot].p + d.n0,
+                  root[slot].p + 6 * (size_t)j, leaf0[slot].p + 6 * (size_t)j, okeys, order, childA, childB,
+                  parent_int, parent_leaf, box_int, box_leaf, root6, s);
 }
 
 }  // namespace cc
