@@ -143,3 +143,54 @@ def test_batch_step_vs_fp64_oracle(gpu):
     say(f"[batch] worst one-step |gpu - fp64| = {worst:.3e}, {nflips} inside flips")
     assert worst <= TOL_STEP
     b.close()
+
+
+def tiny_body(tris, mode, y=0.995, copies=1):
+    """A flat body of a few triangles under the cloth, one capsule covering
+    every vertex (so it can move), with the default sway and breathing."""
+    from simulator_amd import scene as S
+    v = np.array([[0.2, y, -0.3], [0.8, y, -0.3], [0.5, y, 0.3], [0.9, y - 0.05, 0.4]], np.float32)
+    t = np.tile(np.asarray(tris, np.uint32).reshape(-1, 3), (copies, 1))
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
+    caps = {"vstart": np.array([0, 4], np.uint32), "p0": np.array([[0.3, y, 0.0]], np.float32),
+            "p1": np.array([[0.7, y, 0.0]], np.float32), "radius": np.array([0.05], np.float32),
+            "phase": np.array([0.5], np.float32)}
+    return S.Body(v, t, nrm, mode, True, caps, S._motion(9))
+
+
+def test_batch_edge_instances(gpu):
+    """Bodies of 1, 2 and 3 triangles and of 300 identical triangles (every
+    Morton code equal) in both moving modes, a one-vertex cloth without
+    springs, and an instance without a body: each equals its own Simulator,
+    state and body tree, bit for bit."""
+    from simulator_amd import scene as S
+    scs = []
+    for mode in (L.CC_BODY_REBUILD, L.CC_BODY_REFIT):
+        for tris, copies in (([0, 1, 2], 1), ([0, 1, 2, 1, 3, 2], 1), ([0, 1, 2, 1, 3, 2, 0, 3, 2], 1),
+                             ([0, 1, 2], 300)):
+            scs.append(S.grid_scene("edge", 12, 1e-3, 60, tiny_body(tris, mode, copies=copies), box_depth=0.01))
+    one = S.Scene("one", 1, 1, np.array([[0.5, 0.999, 0.0]], np.float32),
+                  [(np.zeros(2, np.uint32), np.zeros(0, L.SPRING_DTYPE))], np.zeros(0, np.uint32),
+                  S.params(), tiny_body([0, 1, 2], L.CC_BODY_REBUILD), 60)
+    scs.append(one)
+    bare = S.grid_scene("bare", 9, 1e-3, 60, None)
+    scs.append(bare)
+    say(f"[batch] {len(scs)} edge instances")
+    b = B.Batch(scs)
+    sims = [Simulator.from_scene(sc) for sc in scs]
+    for nsteps in (1, 16, 7, 20):
+        b.simulate(nsteps)
+        for s in sims:
+            s.simulate(nsteps)
+        b.synchronize()
+        for k, (sc, s) in enumerate(zip(scs, sims)):
+            for g, w, name in zip(b.get_state(k), s.get_state(), ("x_cur", "x_prev", "contact")):
+                np.testing.assert_array_equal(g, w, err_msg=f"instance {k} {name} at frame {b.frame}")
+            if sc.body is not None:
+                v, t = b.body_export(k)
+                np.testing.assert_array_equal(v, s.body_vertices(), err_msg=f"instance {k} body at {b.frame}")
+                assert_bvh_equal(t, s.bvh_export())
+    assert sum(int((np.abs(b.get_state(k)[2]).sum(axis=1) > 0).sum()) for k in range(len(scs))) > 0
+    for s in sims:
+        s.close()
+    b.close()
