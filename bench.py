@@ -29,6 +29,8 @@ WORKLOADS = {
     "garment": "garment: 111x111 cloth, 12,321 vertices, 12-capsule body of 50,688 triangles deformed and "
                "LBVH-rebuilt every frame, dt=1e-3",
     "small": "small: 64x64 cloth, 4,096 vertices, static 5,120-triangle icosphere, dt=1e-3",
+    "reference": "reference: the reference's own scene (main.cpp): tshirt2.obj, 16,773 vertices, 195,534 "
+                 "directed mesh springs, over female.obj, static 13,776 triangles, its constants (dt=1/40)",
 }
 
 
@@ -231,9 +233,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["large", "garment", "small"], default="large",
+    ap.add_argument("--workload", choices=["large", "garment", "small", "reference"], default="large",
                     help="large (default, the headline): 512x512 over 501k refitted triangles; garment: "
-                         "111x111 over 50k triangles rebuilt every frame; small: 64x64 over a static sphere")
+                         "111x111 over 50k triangles rebuilt every frame; small: 64x64 over a static sphere; "
+                         "reference: the reference's tshirt over its female body")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -248,7 +251,9 @@ def main():
     from simulator_amd import scene as S
     from simulator_amd.simulator import Simulator
 
-    sc = {"large": S.large, "garment": S.garment, "small": S.small}[args.workload]()
+    sc = {"large": S.large, "garment": S.garment, "small": S.small,
+          "reference": lambda: S.load_mesh_scene(os.path.join(ROOT, "tests", "golden", "ref_tshirt_female.npz"))
+          }[args.workload]()
     sim = Simulator.from_scene(sc)
     # HIP events around every step graph and update-lane graph, on the
     # streams they run on, read while the following blocks run

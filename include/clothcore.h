@@ -132,6 +132,43 @@ int cc_mesh_vertex_extend(float* verts, uint32_t stride, uint32_t nv, const uint
                           uint32_t nf, const uint32_t* normal_index, const float* normals3,
                           float dist);
 
+/* ------------------------------------------------------------ mesh ingest */
+
+/* ObjLoader::ObjLoader (ObjLoader.cpp:15-216), geometry only: v (vec4,
+   w = 1), vn, vt and the first three corners of each f line, indices
+   0-based as the reference stores them (atoi - 1, unsigned; "f a b c"
+   faces: texture 0, 1, 2 and normal index = vertex index; the reference's
+   reading of "v//n" in the second and third corners is kept).  A corner
+   without '/' in a "v/t/n" face is an error (undefined in the reference).
+   Arrays are malloc'ed; release with cc_obj_free. */
+typedef struct cc_obj {
+  uint32_t nv, nvn, nvt, nf;
+  float* verts4;      /* [4 nv]  */
+  float* normals3;    /* [3 nvn] */
+  float* tex2;        /* [2 nvt] */
+  uint32_t* vidx;     /* [3 nf] vertex index per corner  */
+  uint32_t* tidx;     /* [3 nf] texture index per corner */
+  uint32_t* nidx;     /* [3 nf] normal index per corner  */
+} cc_obj;
+int cc_obj_read(const char* path, cc_obj* out);
+void cc_obj_free(cc_obj* obj);
+
+/* Mesh::unified (Mesh.cpp:17-38): per-vertex normals / texture coordinates
+   from the face corners, the last corner naming a vertex wins; out-of-range
+   indices (the reference reads past its arrays) leave the earlier value or
+   zero.  Either output may be NULL; sizes 3 nv and 2 nv floats. */
+int cc_mesh_unify(const cc_obj* obj, float* normals3, float* tex2);
+
+/* Mesh::get_center / scale / translate / rotation (Mesh.cpp:40-92,
+   119-131) on vec4 vertices in place: center = float sums in vertex order
+   / n; scale about the center; rotation by degrees about the center, axis
+   0 = x, 1 = y, 2 = z, with the reference's matrices (angle * 3.1415 / 180,
+   glm column order). */
+int cc_mesh_center(const float* verts4, uint32_t n, float center[3]);
+int cc_mesh_scale(float* verts4, uint32_t n, float s);
+int cc_mesh_translate(float* verts4, uint32_t n, float x, float y, float z);
+int cc_mesh_rotate(float* verts4, uint32_t n, float degrees, int axis);
+
 /* --------------------------------------------------------------- simulator */
 
 typedef struct cc_sim cc_sim;

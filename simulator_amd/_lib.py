@@ -60,6 +60,13 @@ class cc_cloth_desc(C.Structure):
                 ("row", C.POINTER(C.c_uint32) * 3), ("col", C.POINTER(cc_spring) * 3)]
 
 
+class cc_obj(C.Structure):
+    _fields_ = [("nv", C.c_uint32), ("nvn", C.c_uint32), ("nvt", C.c_uint32), ("nf", C.c_uint32),
+                ("verts4", C.POINTER(C.c_float)), ("normals3", C.POINTER(C.c_float)),
+                ("tex2", C.POINTER(C.c_float)), ("vidx", C.POINTER(C.c_uint32)),
+                ("tidx", C.POINTER(C.c_uint32)), ("nidx", C.POINTER(C.c_uint32))]
+
+
 class cc_body_desc(C.Structure):
     _fields_ = [("nv", C.c_uint32), ("verts", C.POINTER(C.c_float)), ("stride", C.c_uint32),
                 ("nf", C.c_uint32), ("tris", C.POINTER(C.c_uint32)), ("mode", C.c_uint32),
@@ -126,6 +133,13 @@ PROTOTYPES = {
     "cc_adj_destroy": (C.c_int, [P]),
     "cc_adj_forward": (C.c_int, [P, P, P, P, P]),
     "cc_adj_backward": (C.c_int, [P, P, P, P, P]),
+    "cc_obj_read": (C.c_int, [C.c_char_p, P]),
+    "cc_obj_free": (None, [P]),
+    "cc_mesh_unify": (C.c_int, [P, P, P]),
+    "cc_mesh_center": (C.c_int, [P, C.c_uint32, P]),
+    "cc_mesh_scale": (C.c_int, [P, C.c_uint32, C.c_float]),
+    "cc_mesh_translate": (C.c_int, [P, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "cc_mesh_rotate": (C.c_int, [P, C.c_uint32, C.c_float, C.c_int]),
     "cc_batch_create": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
     "cc_batch_destroy": (C.c_int, [P]),
     "cc_batch_step": (C.c_int, [P, C.c_int, P]),

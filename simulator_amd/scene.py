@@ -181,3 +181,45 @@ def adjoint_scene(side=128, steps=200, k=(1000.0, 500.0, 100.0), damp=-0.02, res
     """128x128 cloth over the 5120-triangle icosphere (differentiable workload)."""
     return grid_scene("adjoint", side, 1e-3, steps, sphere_body(), k=k, damp=damp,
                       rest_scale=rest_scale)
+
+
+def reference_scene(cloth_obj, body_obj, steps=200):
+    """The reference's own scene (main.cpp:24-35): tshirt2.obj rotated 90
+    degrees about x and -4 about z, scaled 3.33 about its centre and moved by
+    (0, 1.98, 0.02), over female.obj scaled 3.226 and moved by (0, 1.8, 0);
+    structure and bend springs from the mesh (Springs(Mesh*), spring.cpp:
+    82-91), the reference's constants (verlet.cu:13-21), a static body
+    extended by 0.003 along its normals (simulation.cu:127-138), no pins."""
+    from .simulator import Mesh, X, Z
+    cloth = Mesh.from_obj(cloth_obj)
+    cloth.rotation(90, X)
+    cloth.rotation(-4, Z)
+    cloth.scale(3.33)
+    cloth.translate(0, 1.98, 0.02)
+    body = Mesh.from_obj(body_obj)
+    body.scale(3.226)
+    body.translate(0, 1.8, 0)
+    return mesh_scene("reference", cloth.vertices[:, :3], cloth.faces, body.vertices[:, :3], body.faces,
+                      body.normals, steps)
+
+
+def mesh_scene(name, cloth_verts, cloth_faces, body_verts, body_faces, body_normals, steps=200):
+    """A cloth mesh over a static body mesh with the reference's constants and
+    its mesh springs (structure, then bend)."""
+    pos = np.ascontiguousarray(cloth_verts, np.float32)
+    v4 = np.concatenate([pos, np.ones((pos.shape[0], 1), np.float32)], axis=1)
+    s, b = springs_from_mesh(v4, np.asarray(cloth_faces, np.uint32))
+    body = Body(np.ascontiguousarray(body_verts, np.float32), np.asarray(body_faces, np.uint32),
+                np.ascontiguousarray(body_normals, np.float32), L.CC_BODY_STATIC, True)
+    sc = Scene(name, 0, 0, pos, [s, b], np.zeros(0, np.uint32), L.default_params(), body, steps)
+    sc.faces = np.asarray(cloth_faces, np.uint32)
+    return sc
+
+
+def load_mesh_scene(npz_path, name="reference", steps=200):
+    """mesh_scene from arrays saved as cloth_verts / cloth_faces / body_verts /
+    body_faces / body_normals (e.g. tests/golden/ref_tshirt_female.npz, the
+    reference's own scene)."""
+    f = np.load(npz_path)
+    return mesh_scene(name, f["cloth_verts"], f["cloth_faces"], f["body_verts"], f["body_faces"],
+                      f["body_normals"], steps)

@@ -15,28 +15,99 @@ from . import _lib as L
 from . import scene as S
 
 
+class ObjLoader:
+    """ObjLoader (src/ObjLoader.h:19-38, ObjLoader.cpp:15-216), geometry
+    only, through cc_obj_read: vertices (n, 4), normals (nvn, 3), tex
+    (nvt, 2) and faces as vertex / tex / normal index triples (0-based,
+    unsigned as the reference stores them)."""
+
+    def __init__(self, file):
+        o = L.cc_obj()
+        L.check(L.lib().cc_obj_read(str(file).encode(), C.byref(o)))
+        try:
+            def arr(ptr, n, k, dt):
+                return np.ctypeslib.as_array(ptr, shape=(n * k,)).astype(dt).reshape(n, k) if n else \
+                    np.zeros((0, k), dt)
+            self.obj_file = str(file)
+            self.vertices = arr(o.verts4, o.nv, 4, np.float32)
+            self.normals = arr(o.normals3, o.nvn, 3, np.float32)
+            self.tex = arr(o.tex2, o.nvt, 2, np.float32)
+            self.vertex_index = arr(o.vidx, o.nf, 3, np.uint32)
+            self.tex_index = arr(o.tidx, o.nf, 3, np.uint32)
+            self.normal_index = arr(o.nidx, o.nf, 3, np.uint32)
+            # per-vertex normals / uv as Mesh::unified makes them (None when a
+            # face names a vertex the file does not have)
+            self.unified_normals = self.unified_tex = None
+            if not o.nf or int(self.vertex_index.max()) < o.nv:
+                self.unified_normals = np.zeros((o.nv, 3), np.float32)
+                self.unified_tex = np.zeros((o.nv, 2), np.float32)
+                L.check(L.lib().cc_mesh_unify(C.byref(o), L.ptr(self.unified_normals),
+                                              L.ptr(self.unified_tex)))
+        finally:
+            L.lib().cc_obj_free(C.byref(o))
+
+
+X, Y, Z = 0, 1, 2      # direction (Mesh.h:6)
+
+
 class Mesh:
     """The parts of Mesh (src/Mesh.h:8-51) the simulation path reads:
-    vertices as vec4 (w = 1), faces as vertex index triples, normals."""
+    vertices as vec4 (w = 1), faces as vertex index triples, normals; built
+    from arrays or, as Mesh(string) does, from an OBJ file (Mesh.from_obj)."""
 
-    def __init__(self, vertices, faces, normals=None):
+    def __init__(self, vertices, faces, normals=None, normal_index=None):
         v = np.asarray(vertices, np.float32)
         if v.shape[1] == 3:
             v = np.concatenate([v, np.ones((v.shape[0], 1), np.float32)], axis=1)
         self.vertices = np.ascontiguousarray(v)
         self.faces = np.ascontiguousarray(np.asarray(faces, np.uint32).reshape(-1, 3))
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        self.normal_index = None if normal_index is None else np.ascontiguousarray(normal_index, np.uint32)
+
+    @classmethod
+    def from_obj(cls, file):
+        """Mesh(string file) = ObjLoader + Mesh::unified (Mesh.cpp:11-38)."""
+        o = ObjLoader(file)
+        if o.unified_normals is None:
+            raise ValueError(f"{file}: a face names a vertex the file does not have")
+        m = cls(o.vertices, o.vertex_index, o.unified_normals, o.normal_index)
+        m.tex = o.unified_tex
+        return m
+
+    def get_center(self):
+        c = np.zeros(3, np.float32)
+        L.check(L.lib().cc_mesh_center(L.ptr(self.vertices), self.vertices.shape[0], L.ptr(c)))
+        return c
+
+    def scale(self, S):
+        """Mesh::scale (Mesh.cpp:40-50): about the center."""
+        L.check(L.lib().cc_mesh_scale(L.ptr(self.vertices), self.vertices.shape[0], float(S)))
+
+    def translate(self, x, y, z):
+        L.check(L.lib().cc_mesh_translate(L.ptr(self.vertices), self.vertices.shape[0], float(x), float(y),
+                                          float(z)))
+
+    def rotation(self, angle, direction):
+        """Mesh::rotation (Mesh.cpp:61-92): degrees about the center."""
+        L.check(L.lib().cc_mesh_rotate(L.ptr(self.vertices), self.vertices.shape[0], float(angle),
+                                       int(direction)))
 
     @property
     def vertex_indices(self):
         return self.faces.reshape(-1)
 
     def vertex_extend(self, dist):
-        """Mesh::vertex_extend (Mesh.cpp:101-117), per-vertex normals."""
+        """Mesh::vertex_extend (Mesh.cpp:101-117): the normal of the first
+        corner visiting each vertex (its normal index into the per-vertex
+        normals, as the reference indexes them)."""
         if self.normals is None:
             raise ValueError("vertex_extend needs normals")
+        ni = None if self.normal_index is None else L.ptr(self.normal_index)
+        if self.normal_index is not None and self.normal_index.size and \
+                int(self.normal_index.max()) >= self.normals.shape[0]:
+            raise ValueError("vertex_extend: a normal index exceeds the per-vertex normals")
         L.check(L.lib().cc_mesh_vertex_extend(L.ptr(self.vertices), 4, self.vertices.shape[0],
-                                              L.ptr(self.faces), self.faces.shape[0], None,
+                                              L.ptr(self.faces), self.faces.shape[0], ni,
                                               L.ptr(self.normals), float(dist)))
 
 
