@@ -301,7 +301,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded grid cloth and capsule-union body)",
+        "data": ("the reference's own meshes (tshirt2.obj, female.obj) from tests/golden/ref_tshirt_female.npz"
+                 if args.workload == "reference" else "synthetic (seeded grid cloth and capsule-union body)"),
         "config": {"workload": WORKLOADS[args.workload],
                    "vertices": sc.n, "springs_directed": ab["springs"],
                    "body_triangles": int(sc.body.tris.shape[0])},
