@@ -296,6 +296,16 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
    lag = 1 reproduces the reference's use of the previous positions for face
    normals (simulation.cu:221).  uv bytes are left untouched. */
 int cc_sim_set_faces(cc_sim* sim, const uint32_t* faces, uint32_t nf);
+
+/* Picking (Scene::OnMouseDown, scene.cpp:365-436, and
+   find_first_intersect_triangle, Ray.cpp:8-77): the cloth face (the faces of
+   cc_sim_set_faces) that the ray origin + t dir meets with the smallest t,
+   Moller-Trumbore without culling, ties (equal t) to the later face, t not
+   required to be positive -- all as the reference -- and the face's first
+   vertex, the one the reference then drags with update_vertex.  -1 when no
+   face is met.  The reference tests its host mesh's initial positions
+   (never updated by its simulation); this tests the current positions. */
+int cc_sim_pick(cc_sim* sim, const float origin[3], const float dir[3], int32_t* face, int32_t* vertex, float* t);
 int cc_sim_write_vbo(cc_sim* sim, void* dst_dev, size_t bytes, uint32_t n_tex, int lag);
 /* Host copy of the vertex normals of the last cc_sim_write_vbo. */
 int cc_sim_get_normals(cc_sim* sim, float* normals3);

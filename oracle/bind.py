@@ -76,6 +76,8 @@ def olib():
             "or_step_f64": (None, [u32, V, V, V, V, V, V, V, V, V, V, V, V, V, V, V, V, V]),
             "or_collide_points": (None, [u32, V, V, V, V, V, V]),
             "or_body_deform": (None, [u32, V, u32, V, V, V, V, V, V, V]),
+            "or_pick": (C.c_int32, [u32, V, V, V, V, V]),
+            "or_vertex_normals": (None, [u32, V, u32, V, V]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -412,3 +414,23 @@ def adjoint_f64(springs, prm64, pins, traj_p, traj_q, traj_c, dl_dxf):
                           _p(_c(traj_p, np.float64)), _p(_c(traj_q, np.float64)), _p(_c(traj_c, np.float64)),
                           _p(_c(dl_dxf, np.float64)), _p(dp), _p(dv0), _p(dx0))
     return dp, dv0, dx0
+
+
+def pick(verts3, faces, origin, direction):
+    """find_first_intersect_triangle (Ray.cpp:45-77): (face or -1, t)."""
+    v = _c(verts3, np.float32)
+    f = _c(faces, np.uint32)
+    o = _c(origin, np.float32)
+    d = _c(direction, np.float32)
+    t = C.c_float()
+    face = olib().or_pick(f.size // 3, _p(f), _p(v), _p(o), _p(d), C.byref(t))
+    return int(face), float(t.value)
+
+
+def vertex_normals(pos3, faces):
+    """The VBO's vertex normals (verlet.cu:91-116, 163-181)."""
+    p = _c(pos3, np.float32)
+    f = _c(faces, np.uint32)
+    out = np.zeros((p.shape[0], 3), np.float32)
+    olib().or_vertex_normals(p.shape[0], _p(p), f.size // 3, _p(f), _p(out))
+    return out

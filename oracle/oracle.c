@@ -1049,3 +1049,73 @@ void or_body_deform(uint32_t nv, const float* rest3, uint32_t ncaps, const uint3
     out3[3 * (size_t)v + 2] = ((R[6] * Y.x + R[7] * Y.y) + R[8] * Y.z) + T[2];
   }
 }
+
+/* ---------------------------------------------------------------- picking */
+/* rayTriangleIntersect, Moller-Trumbore without culling (Ray.cpp:8-42), in
+   glm's operation order: 1 when the ray meets the triangle (t may be
+   negative: the reference does not test it). */
+static int ray_triangle(v3f o, v3f d, v3f v0, v3f v1, v3f v2, float* t) {
+  v3f v0v1 = vsub(v1, v0), v0v2 = vsub(v2, v0);
+  v3f pvec = {d.y * v0v2.z - v0v2.y * d.z, d.z * v0v2.x - v0v2.z * d.x, d.x * v0v2.y - v0v2.x * d.y};
+  float det = vdot(v0v1, pvec);
+  if (fabsf(det) < FLT_EPSILON) return 0;
+  float invDet = 1 / det;
+  v3f tvec = vsub(o, v0);
+  float u = vdot(tvec, pvec) * invDet;
+  if (u < 0 || u > 1) return 0;
+  v3f qvec = {tvec.y * v0v1.z - v0v1.y * tvec.z, tvec.z * v0v1.x - v0v1.z * tvec.x,
+              tvec.x * v0v1.y - v0v1.x * tvec.y};
+  float v = vdot(d, qvec) * invDet;
+  if (v < 0 || u + v > 1) return 0;
+  *t = vdot(v0v2, qvec) * invDet;
+  return 1;
+}
+
+/* find_first_intersect_triangle (Ray.cpp:45-77): the face with the smallest
+   t, ties (equal t, -0 == +0) to the later face -- map<float, int> with
+   isect_points[t] = i -- or -1.  A NaN t is skipped (the reference's map
+   would be inconsistent). */
+int32_t or_pick(uint32_t nf, const uint32_t* faces, const float* verts3, const float origin[3],
+                const float dir[3], float* t_out) {
+  v3f o = f3(origin), d = f3(dir);
+  int32_t best = -1;
+  float bt = 0.0f;
+  for (uint32_t f = 0; f < nf; ++f) {
+    const uint32_t* x = faces + 3 * (size_t)f;
+    float t;
+    if (!ray_triangle(o, d, f3(verts3 + 3 * (size_t)x[0]), f3(verts3 + 3 * (size_t)x[1]),
+                      f3(verts3 + 3 * (size_t)x[2]), &t))
+      continue;
+    if (t != t) continue;
+    if (best < 0 || t <= bt) { best = (int32_t)f; bt = t; }
+  }
+  if (t_out) *t_out = bt;
+  return best;
+}
+
+/* ------------------------------------------------------------ vbo normals */
+/* compute_face_normal + get_vertex_adjface + compute_vbo_normal
+   (verlet.cu:91-116, 163-181, simulation.cu:152-180): face normal =
+   normalize(cross(p1 - p0, p2 - p0)); vertex normal = normalize(sum of the
+   normals of the faces naming it, in face order, once per corner). */
+void or_vertex_normals(uint32_t n, const float* pos3, uint32_t nf, const uint32_t* faces, float* out3) {
+  float* fn = (float*)malloc(sizeof(float) * 3 * (nf ? nf : 1));
+  for (uint32_t f = 0; f < nf; ++f) {
+    const uint32_t* x = faces + 3 * (size_t)f;
+    v3f a = f3(pos3 + 3 * (size_t)x[0]), b = f3(pos3 + 3 * (size_t)x[1]), c = f3(pos3 + 3 * (size_t)x[2]);
+    v3f nn = vnorm(vcross(vsub(b, a), vsub(c, a)));
+    fn[3 * (size_t)f] = nn.x; fn[3 * (size_t)f + 1] = nn.y; fn[3 * (size_t)f + 2] = nn.z;
+  }
+  for (uint32_t i = 0; i < n; ++i) { out3[3 * (size_t)i] = out3[3 * (size_t)i + 1] = out3[3 * (size_t)i + 2] = 0.0f; }
+  /* sums in face order per vertex: walk the faces once */
+  for (uint32_t f = 0; f < nf; ++f)
+    for (int j = 0; j < 3; ++j) {
+      float* o = out3 + 3 * (size_t)faces[3 * (size_t)f + j];
+      o[0] = o[0] + fn[3 * (size_t)f]; o[1] = o[1] + fn[3 * (size_t)f + 1]; o[2] = o[2] + fn[3 * (size_t)f + 2];
+    }
+  for (uint32_t i = 0; i < n; ++i) {
+    v3f nn = vnorm(f3(out3 + 3 * (size_t)i));
+    out3[3 * (size_t)i] = nn.x; out3[3 * (size_t)i + 1] = nn.y; out3[3 * (size_t)i + 2] = nn.z;
+  }
+  free(fn);
+}

@@ -226,6 +226,13 @@ void or_body_deform(uint32_t nv, const float* rest3, uint32_t ncaps, const uint3
                     const float* cap_p0, const float* cap_p1, const float* breath,
                     const float R[9], const float T[3], float* out3);
 
+/* find_first_intersect_triangle (Ray.cpp:45-77) over faces of verts3. */
+int32_t or_pick(uint32_t nf, const uint32_t* faces, const float* verts3, const float origin[3],
+                const float dir[3], float* t_out);
+
+/* cloth vertex normals of the VBO (verlet.cu:91-116, 163-181). */
+void or_vertex_normals(uint32_t n, const float* pos3, uint32_t nf, const uint32_t* faces, float* out3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,6 +140,7 @@ PROTOTYPES = {
     "cc_mesh_scale": (C.c_int, [P, C.c_uint32, C.c_float]),
     "cc_mesh_translate": (C.c_int, [P, C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "cc_mesh_rotate": (C.c_int, [P, C.c_uint32, C.c_float, C.c_int]),
+    "cc_sim_pick": (C.c_int, [P, P, P, P, P, P]),
     "cc_batch_create": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
     "cc_batch_destroy": (C.c_int, [P]),
     "cc_batch_step": (C.c_int, [P, C.c_int, P]),

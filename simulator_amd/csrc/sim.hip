@@ -10,6 +10,7 @@
 //   rec       8 B per directed spring {end | type << 30, rest length}
 #include <algorithm>
 #include <cstdlib>
+#include <cfloat>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -154,6 +155,54 @@ void Sim::launch_step(const float* in, float* out, float* contact_io, const Step
 }  // namespace cc
 
 using namespace cc;
+
+namespace cc {
+
+// rayTriangleIntersect (Ray.cpp:8-42, Moller-Trumbore without culling) in
+// glm's operation order, for the picking kernel.
+__device__ inline bool ray_triangle(f3 o, f3 d, f3 v0, f3 v1, f3 v2, float& t) {
+  const f3 v0v1 = v1 - v0, v0v2 = v2 - v0;
+  const f3 pvec = cross(d, v0v2);
+  const float det = dot(v0v1, pvec);
+  if (fabsf(det) < FLT_EPSILON) return false;
+  const float invDet = 1 / det;
+  const f3 tvec = o - v0;
+  const float u = dot(tvec, pvec) * invDet;
+  if (u < 0 || u > 1) return false;
+  const f3 qvec = cross(tvec, v0v1);
+  const float v = dot(d, qvec) * invDet;
+  if (v < 0 || u + v > 1) return false;
+  t = dot(v0v2, qvec) * invDet;
+  return true;
+}
+
+// find_first_intersect_triangle (Ray.cpp:45-77) over all faces at once: key
+// = (t as an ordered integer, -0 == +0) << 32 | (2^32 - 1 - face), so the
+// smallest key is the smallest t and, among equal t, the later face -- the
+// entry map<float, int> keeps.  One 64-bit atomic minimum per wave.
+__global__ __launch_bounds__(256) void k_pick(const float* __restrict__ state, const uint32_t* __restrict__ faces,
+                                              uint32_t nf, f3 o, f3 d, unsigned long long* __restrict__ out) {
+  unsigned long long best = ~0ull;
+  for (uint32_t f = blockIdx.x * 256 + threadIdx.x; f < nf; f += gridDim.x * 256) {
+    const uint32_t* x = faces + 3 * (size_t)f;
+    float t;
+    if (!ray_triangle(o, d, ld3(state + 6 * (size_t)x[0]), ld3(state + 6 * (size_t)x[1]),
+                      ld3(state + 6 * (size_t)x[2]), t) ||
+        t != t)
+      continue;
+    const uint32_t b = __float_as_uint(t + 0.0f);
+    const uint32_t ord = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    const unsigned long long key = ((unsigned long long)ord << 32) | (0xFFFFFFFFu - f);
+    best = key < best ? key : best;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long other = __shfl_xor(best, off, 64);
+    best = other < best ? other : best;
+  }
+  if ((threadIdx.x & 63u) == 0 && best != ~0ull) atomicMin(out, best);
+}
+
+}  // namespace cc
 
 void cc::merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec) {
   const uint32_t n = c->n;
@@ -581,6 +630,38 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
 }
 
 // Simulator::get_vertex_adjface (simulation.cu:152-180)
+int cc_sim_pick(cc_sim* sim, const float origin[3], const float dir[3], int32_t* face, int32_t* vertex, float* t) {
+  return guard([&] {
+    CC_REQUIRE(sim && origin && dir, "cc_sim_pick: bad argument");
+    Sim& S = sim->s;
+    CC_REQUIRE(S.nf > 0, "cc_sim_pick: no faces (cc_sim_set_faces)");
+    CC_HIP(hipDeviceSynchronize());
+    DBuf<unsigned long long> res;
+    res.alloc(1);
+    CC_HIP(hipMemsetAsync(res.p, 0xFF, sizeof(unsigned long long), S.stream));
+    const uint32_t g = std::min<uint32_t>(grid_for(S.nf, 256), 1024);
+    k_pick<<<g, 256, 0, S.stream>>>(S.state[S.cur].p, S.faces.p, S.nf, ld3(origin), ld3(dir), res.p);
+    CC_HIP(hipGetLastError());
+    unsigned long long key = 0;
+    CC_HIP(hipMemcpyAsync(&key, res.p, sizeof(key), hipMemcpyDeviceToHost, S.stream));
+    CC_HIP(hipStreamSynchronize(S.stream));
+    int32_t f = -1, v = -1;
+    float tt = 0.0f;
+    if (key != ~0ull) {
+      f = (int32_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
+      const uint32_t ord = (uint32_t)(key >> 32);
+      const uint32_t b = (ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord;
+      std::memcpy(&tt, &b, 4);
+      uint32_t v0 = 0;     // OnMouseDown selects the face's first vertex (scene.cpp:431-434)
+      CC_HIP(hipMemcpy(&v0, S.faces.p + 3 * (size_t)f, 4, hipMemcpyDeviceToHost));
+      v = (int32_t)v0;
+    }
+    if (face) *face = f;
+    if (vertex) *vertex = v;
+    if (t) *t = tt;
+  });
+}
+
 int cc_sim_set_faces(cc_sim* sim, const uint32_t* f3i, uint32_t nf) {
   return guard([&] {
     CC_REQUIRE(sim && f3i && nf > 0, "cc_sim_set_faces: bad argument");

@@ -223,3 +223,14 @@ def load_mesh_scene(npz_path, name="reference", steps=200):
     f = np.load(npz_path)
     return mesh_scene(name, f["cloth_verts"], f["cloth_faces"], f["body_verts"], f["body_faces"],
                       f["body_normals"], steps)
+
+
+def grid_faces(nx, nz):
+    """Two triangles per cell of an nx x nz grid cloth (vertex (i, j) =
+    j * nx + i), as the faces of its VBO / picking mesh."""
+    f = []
+    for j in range(nz - 1):
+        for i in range(nx - 1):
+            a = j * nx + i
+            f += [(a, a + nx, a + 1), (a + 1, a + nx, a + nx + 1)]
+    return np.asarray(f, np.uint32).reshape(-1, 3)

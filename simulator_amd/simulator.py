@@ -351,6 +351,16 @@ class Simulator:
         f = np.ascontiguousarray(faces, np.uint32)
         L.check(L.lib().cc_sim_set_faces(self._h, L.ptr(f), f.size // 3))
 
+    def pick(self, origin, direction):
+        """(face, vertex, t) of the first cloth face along the ray
+        (Scene::OnMouseDown + find_first_intersect_triangle), face = -1 if
+        none; needs set_faces."""
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        f, v, t = C.c_int32(), C.c_int32(), C.c_float()
+        L.check(L.lib().cc_sim_pick(self._h, L.ptr(o), L.ptr(d), C.byref(f), C.byref(v), C.byref(t)))
+        return f.value, v.value, t.value
+
     def get_normals(self):
         nrm = np.zeros((self.n, 3), np.float32)
         L.check(L.lib().cc_sim_get_normals(self._h, L.ptr(nrm)))

@@ -1,4 +1,5 @@
 """Shared helpers for the parity tests (oracle on the CPU vs the HIP path)."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -9,6 +10,31 @@ from simulator_amd import scene as S
 from simulator_amd.simulator import Simulator
 
 BVH_FIELDS = ("keys", "order", "childA", "childB", "parent_int", "parent_leaf", "box_int", "box_leaf")
+
+
+class DeviceBuffer:
+    """Device memory for a test, from the HIP runtime the library links
+    (hipMalloc / hipMemcpy / hipFree); filled from and read into numpy."""
+
+    def __init__(self, host):
+        L.lib()
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = self.host.nbytes
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(self.nbytes)) == 0
+        assert self.hip.hipMemcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), 1) == 0
+
+    def read(self):
+        out = np.empty_like(self.host)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes), 2) == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
 
 
 def say(*a):
