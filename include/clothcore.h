@@ -271,6 +271,17 @@ int cc_bvh_rebuild(cc_sim* sim);
 int cc_bvh_refit(cc_sim* sim);
 int cc_bvh_size(cc_sim* sim, uint32_t* leaves);
 
+/* Traversal form of a static or refitted body (no counterpart in the
+   reference, whose D_BVH::intersect, cudaBRT.cu:239-285, walks the binary
+   tree).  Width 4 (the default) walks a four-wide form refolded from the
+   binary tree after every refit, entries in the binary walk's order of
+   consideration, so first hits and contacts are the same; width 2 walks the
+   binary tree.  Rebuilt bodies, and trees whose root paths could defer more
+   than 64 entries, always walk the binary tree.  cc_bvh_width reports the
+   form in use, the wide node count and the deferral bound of the tree. */
+int cc_sim_set_bvh_width(cc_sim* sim, int width);
+int cc_bvh_width(cc_sim* sim, int* width, uint32_t* wide_nodes, uint32_t* stack_bound);
+
 /* Export the LBVH in the reference encoding (cudaBRT.h:10-33): childA/B
    > 0 internal v-1, < 0 leaf -(v+1); parent v+1, 0 = root.  keys are the
    sorted (morton << 32 | primitive) keys, order the primitive of each leaf,

@@ -122,6 +122,8 @@ PROTOTYPES = {
     "cc_bvh_rebuild": (C.c_int, [P]),
     "cc_bvh_refit": (C.c_int, [P]),
     "cc_bvh_size": (C.c_int, [P, C.POINTER(u32)]),
+    "cc_sim_set_bvh_width": (C.c_int, [P, C.c_int]),
+    "cc_bvh_width": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "cc_bvh_export": (C.c_int, [P, P, P, P, P, P, P, P, P, P]),
     "cc_collide_points": (C.c_int, [P, P, u32, P, P, P, P]),
     "cc_sim_set_faces": (C.c_int, [P, P, u32]),

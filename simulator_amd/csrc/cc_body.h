@@ -87,6 +87,8 @@ struct Slot {
   DBuf<float> root;        // [6] root box (sentinel U all)
   DBuf<float> leaf0_box;   // [6] box of leaf 0 when m == 1
   DBuf<float> motion;      // [12 + ncaps] motion parameters of the frame
+  DBuf<float4> wide;       // [8 * nwide] four-wide nodes (static and refitted bodies): four
+                           // 32-byte entries {min xyz, max x | max yz, link, 0}
 };
 
 // Scratch of one update lane.  A full lane can build a tree; a refit-only
@@ -155,6 +157,17 @@ struct Body {
   std::vector<float> h_phase;       // [ncaps] host copy for the motion
   MotionParams motion;
 
+  // Four-wide form of a static or refitted body's tree (fixed topology):
+  // per wide node, the binary records holding its entries' boxes (record *
+  // 2 + side, ~0 for an empty entry) and its links (wide index >= 0, ~leaf).
+  uint32_t nwide = 0;
+  uint32_t wide_stack = 0;          // most entries a traversal can defer
+  bool wide_enabled = true;
+  DBuf<uint4> w_src;
+  DBuf<int4> w_link;
+  DBuf<uint2> w_dst;                // [2 * (m - 1)] binary child slot -> {wide entry (record * 4 +
+                                    // entry), its link}, entry ~0 for the slots of folded nodes
+
   int nslots = 1, nlanes = 1;
   Slot slot[MAX_SLOTS];
   Lane lane[MAX_LANES];
@@ -175,6 +188,7 @@ struct Body {
     v.verts = slot[k].verts.p;
     v.leaf0_box = slot[k].leaf0_box.p;
     v.m = (mode == CC_BODY_NONE) ? 0 : m;
+    v.wide = (nwide && wide_enabled) ? slot[k].wide.p : nullptr;
     return v;
   }
 
@@ -189,9 +203,15 @@ struct Body {
   void build_levels(int k, int l, hipStream_t s);   // bottom-up tree + boxes from lane l's keys and boxes
   void prim_boxes_of(int k, int l, hipStream_t s);  // lane l's expanded boxes of slot k's pose
   void top_setup(int k, int l, hipStream_t s);  // refit description of slot k's topology into lane l
-  void refit(int k, int l, hipStream_t s);      // boxes of slot k
+  // boxes of slot k: the four-wide form's only, when the body is walked
+  // through it, unless `binary`
+  void refit(int k, int l, hipStream_t s, bool binary = false);
   void update(int k, int l, uint64_t frame, hipStream_t s);   // per-mode update of slot k to `frame`
   void share_topology(int from, hipStream_t s); // copy slot `from`'s tree to every other slot
+  // Four-wide form: the tables from slot k's tree and slot k's wide records
+  // (host, synchronous; keeps the binary form when the stack bound fails);
+  // later refits write the boxes through w_dst.
+  void wide_setup(int k, hipStream_t s);
 
   // host export in the reference encoding (lane 0 holds the keys and leaf
   // parents of slot k's tree)

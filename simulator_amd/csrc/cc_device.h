@@ -83,15 +83,68 @@ __device__ inline int32_t trav_step(f3 p, float4 q0, float4 q1, float4 q2, int4 
   return st.pop(next) ? STEP_POP : STEP_MISS;
 }
 
+__device__ inline bool in_box(f3 p, float lx, float ly, float lz, float hx, float hy, float hz) {
+  return (p.x >= lx) & (p.x <= hx) & (p.y >= ly) & (p.y <= hy) & (p.z >= lz) & (p.z <= hz);
+}
+
+// The same first hit through the four-wide nodes.  A binary node X with
+// children A, B considers, in this order: A if it is a leaf, B if it is a
+// leaf, then the subtree of A, then the subtree of B (trav_step).  A wide
+// node stores the entries of X and of its internal children in exactly that
+// order of consideration (leaf children of X; then per internal child c:
+// c's leaf children, then c's internal children), so the first hit is the
+// first entry whose box holds p: a leaf is returned, an internal entry is
+// descended with the later hits deferred behind it, and a deferred leaf is
+// returned when popped.  The boxes of the folded nodes are not needed: a
+// node's box is the union of its children's, so it holds p whenever a box
+// below it does.  Entries past the count have an empty box (+inf, -inf).
+// A record is four 32-byte entries {min xyz, max x | max yz, link, 0}.
+// The set-up keeps the wide form only when no root path defers more than
+// 64 entries (sum of entries - 1 over the path), so the stack cannot
+// overflow.
+__device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill) {
+  TravStack st;
+  st.spill = spill;
+  int32_t node = 0;
+  for (uint32_t guard = 0; guard < b.m; ++guard) {
+    // entry e = q[2e], q[2e + 1]: min xyz, max x | max yz, link
+    const float4* nd = b.wide + 8 * (size_t)node;
+    const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6],
+                 q7 = nd[7];
+    ++visits;
+    uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
+                    (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
+                    (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
+                    (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
+    const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
+                  c3 = __float_as_int(q7.z);
+    if (mask == 0) {
+      if (!st.pop(node)) return -1;
+      if (node < 0) return ~node;
+      continue;
+    }
+    const uint32_t first = __builtin_ctz(mask);
+    const int32_t c = first == 0 ? c0 : first == 1 ? c1 : first == 2 ? c2 : c3;
+    if (c < 0) return ~c;
+    mask &= mask - 1;
+    if ((mask & 8u) && !st.push(c3)) return -1;
+    if ((mask & 4u) && !st.push(c2)) return -1;
+    if ((mask & 2u) && !st.push(c1)) return -1;
+    node = c;
+  }
+  return -1;
+}
+
 // First-hit traversal (D_BVH::intersect, cudaBRT.cu:239-285): one node
 // record (64 bytes, both child boxes and links) per memory round trip.  The
 // stack never holds more entries than the node depth, and the depth of a
 // Karras tree over unique 64-bit keys is below 64, so 64 entries cannot
 // overflow.  Returns the sorted leaf index, or -1.  *visits counts internal
-// nodes processed.
+// nodes processed (wide nodes when the view has them).
 __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill) {
   if (b.m == 0) return -1;
   if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
+  if (b.wide) return wide_first_hit(b, p, visits, spill);
   TravStack st;
   st.spill = spill;
   int32_t node = 0;

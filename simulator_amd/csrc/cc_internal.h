@@ -120,6 +120,10 @@ struct BvhView {
   const float* verts;       // body vertices, float3
   const float* leaf0_box;   // [6] box of leaf 0 when m == 1
   uint32_t m;
+  // The same tree with every second level folded into its parent: 128-byte
+  // nodes of up to four entries (four boxes, four links); nullptr when the
+  // body is traversed through `nodes`.
+  const float4* wide = nullptr;
 };
 
 // --------------------------------------------------- FP32 math, glm order

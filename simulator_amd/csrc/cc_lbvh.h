@@ -19,6 +19,16 @@ __device__ inline void unite(float* mn, float* mx, const float* b6) {
   }
 }
 
+// A box of the four-wide form: entry d = {record * 4 + entry, link}, ~0 for
+// none.  An entry is one 32-byte sector {min xyz, max x | max yz, link, 0}
+// written whole by one thread (two 16-byte stores).
+__device__ inline void put_wide_at(float4* wide, uint2 d, const float* b6) {
+  if (d.x == ~0u) return;
+  float4* f = wide + 2 * (size_t)d.x;
+  f[0] = make_float4(b6[0], b6[1], b6[2], b6[3]);
+  f[1] = make_float4(b6[4], b6[5], __int_as_float((int32_t)d.y), 0.0f);
+}
+
 __device__ inline void box_sentinel(float* mn, float* mx) {   // bbox.h:17-22
   mn[0] = mn[1] = mn[2] = 100.0f;
   mx[0] = mx[1] = mx[2] = -100.0f;

@@ -427,7 +427,9 @@ void Body::build(int k, int l, hipStream_t s) {
     CC_HIP(hipGetLastError());
   }
   top_setup(k, l, s);
+  nwide = 0;             // a new topology: the wide table follows the refit
   refit(k, l, s);
+  wide_setup(k, s);
 }
 
 void Body::share_topology(int from, hipStream_t s) {
@@ -439,6 +441,7 @@ void Body::share_topology(int from, hipStream_t s) {
     CC_HIP(hipMemcpyAsync(S.nodes.p, F.nodes.p, F.nodes.n * sizeof(Node), hipMemcpyDeviceToDevice, s));
     CC_HIP(hipMemcpyAsync(S.root.p, F.root.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
     CC_HIP(hipMemcpyAsync(S.leaf0_box.p, F.leaf0_box.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (nwide) CC_HIP(hipMemcpyAsync(S.wide.p, F.wide.p, F.wide.n * sizeof(float4), hipMemcpyDeviceToDevice, s));
   }
 }
 

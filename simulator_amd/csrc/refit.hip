@@ -203,7 +203,8 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     const float* __restrict__ verts, const uint4* __restrict__ leaf_tri, const int32_t* __restrict__ leaf_parent,
     Node* __restrict__ nodes, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
     const uint8_t* __restrict__ lflag, const uint32_t* __restrict__ chunk_h, const int32_t* __restrict__ top_pos,
-    float* __restrict__ top_child, float* __restrict__ root_out, float* __restrict__ leaf0_box) {
+    float* __restrict__ top_child, float* __restrict__ root_out, float* __restrict__ leaf0_box,
+    float4* __restrict__ wide, const uint2* __restrict__ wdst) {
   __shared__ float lbox[RC_CHUNK][6];
   __shared__ float nbox[RC_CHUNK][6];
   const int32_t c0 = (int32_t)(blockIdx.x * RC_CHUNK);
@@ -223,6 +224,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
   int4 L[RC_PER_THREAD];
   uint8_t fl[RC_PER_THREAD], lf[RC_PER_THREAD];
   int32_t lpar[RC_PER_THREAD];
+  uint4 wd[RC_PER_THREAD];         // wide entries mirroring the node's two child slots
 #pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const int32_t l = c0 + r * RC_THREADS + (int32_t)tid;
@@ -232,6 +234,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     lpar[r] = leaf_parent[lc];
     L[r] = reinterpret_cast<const int4*>(nodes + xc)[3];
     fl[r] = nflag[xc];
+    wd[r] = wide ? reinterpret_cast<const uint4*>(wdst)[xc] : make_uint4(~0u, 0u, ~0u, 0u);
   }
 #pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
@@ -293,10 +296,15 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     if (x < i1 && (fl[r] & H_MASK)) {
       const float* a = L[r].x < 0 ? lbox[~L[r].x - c0] : nbox[L[r].x - c0];
       const float* b = L[r].y < 0 ? lbox[~L[r].y - c0] : nbox[L[r].y - c0];
-      float4* d = reinterpret_cast<float4*>(nodes + x);
-      d[0] = make_float4(a[0], a[1], a[2], a[3]);
-      d[1] = make_float4(a[4], a[5], b[0], b[1]);
-      d[2] = make_float4(b[2], b[3], b[4], b[5]);
+      if (wide) {                         // the four-wide form only
+        put_wide_at(wide, make_uint2(wd[r].x, wd[r].y), a);
+        put_wide_at(wide, make_uint2(wd[r].z, wd[r].w), b);
+      } else {
+        float4* d = reinterpret_cast<float4*>(nodes + x);
+        d[0] = make_float4(a[0], a[1], a[2], a[3]);
+        d[1] = make_float4(a[4], a[5], b[0], b[1]);
+        d[2] = make_float4(b[2], b[3], b[4], b[5]);
+      }
       if (fl[r] & BELOW_TOP) {
         const int32_t p = L[r].z;
         const float* bx = nbox[x - c0];
@@ -304,7 +312,10 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
           for (int c = 0; c < 6; ++c) root_out[c] = bx[c];
         } else {
           const bool isA = (fl[r] & IS_A) != 0;
-          put6(isA ? nodes[p].a_min : nodes[p].b_min, bx, bx + 3);
+          if (wide)
+            put_wide_at(wide, wdst[2 * (size_t)p + (isA ? 0 : 1)], bx);
+          else
+            put6(isA ? nodes[p].a_min : nodes[p].b_min, bx, bx + 3);
           put6(top_child + 12 * (size_t)tpos_n[r] + (isA ? 0 : 6), bx, bx + 3);
         }
       }
@@ -313,7 +324,10 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     if (l < c1 && (lf[r] & BELOW_TOP)) {
       const int32_t p = lpar[r];
       const bool isA = (lf[r] & IS_A) != 0;
-      put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
+      if (wide)
+        put_wide_at(wide, wdst[2 * (size_t)p + (isA ? 0 : 1)], lbox[l - c0]);
+      else
+        put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
       put6(top_child + 12 * (size_t)tpos_l[r] + (isA ? 0 : 6), lbox[l - c0], lbox[l - c0] + 3);
     }
   }
@@ -376,7 +390,8 @@ __global__ __launch_bounds__(TOP_THREADS) void k_top_sort(int32_t* __restrict__ 
 template <bool LDS>
 __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t H, const uint32_t* __restrict__ lev,
                                  const int4* __restrict__ links, const float* __restrict__ top_child,
-                                 float* __restrict__ root_out, float* box, int2* lnk, int32_t* lsl) {
+                                 float* __restrict__ root_out, float* box, int2* lnk, int32_t* lsl,
+                                 float4* __restrict__ wide, const uint2* __restrict__ wdst) {
   const uint32_t tid = threadIdx.x;
   // Entry boxes from the non-top children's boxes.  TOP_LOAD entries per
   // thread per round, every load of a round issued before any use (indices
@@ -422,9 +437,13 @@ __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t 
   }
   for (uint32_t i = tid; i < T; i += TOP_REFIT_THREADS) {
     const int32_t s = lsl[i];
-    float* d = s < 0 ? root_out : (s & 1 ? nodes[s >> 1].b_min : nodes[s >> 1].a_min);
     const float* b = box + 6 * (size_t)i;
-    for (int c = 0; c < 6; ++c) d[c] = b[c];
+    if (wide && s >= 0) {
+      put_wide_at(wide, wdst[s], b);
+    } else {
+      float* d = s < 0 ? root_out : (s & 1 ? nodes[s >> 1].b_min : nodes[s >> 1].a_min);
+      for (int c = 0; c < 6; ++c) d[c] = b[c];
+    }
   }
 }
 
@@ -434,7 +453,9 @@ __global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restric
                                                                  const float* __restrict__ top_child,
                                                                  const uint32_t* __restrict__ top_count,
                                                                  const uint32_t* __restrict__ top_hmax,
-                                                                 float* root_out, float* __restrict__ scratch) {
+                                                                 float* root_out, float* __restrict__ scratch,
+                                                                 float4* __restrict__ wide,
+                                                                 const uint2* __restrict__ wdst) {
   __shared__ float sbox[TOP_MAX_LDS * 6];
   __shared__ int2 slnk[TOP_MAX_LDS];
   __shared__ int32_t ssl[TOP_MAX_LDS];
@@ -444,14 +465,14 @@ __global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restric
   for (uint32_t h = threadIdx.x; h <= H + 1; h += TOP_REFIT_THREADS) slev[h] = lev[h];
   __syncthreads();
   if (T <= (uint32_t)TOP_MAX_LDS) {
-    refit_top_levels<true>(nodes, T, H, slev, links, top_child, root_out, sbox, slnk, ssl);
+    refit_top_levels<true>(nodes, T, H, slev, links, top_child, root_out, sbox, slnk, ssl, wide, wdst);
   } else {
     // more entries than LDS holds: the same levels on global scratch (one
     // workgroup, so the barriers order it)
     float* b = scratch;
     int2* l = reinterpret_cast<int2*>(b + 6 * (size_t)T);
     int32_t* sl = reinterpret_cast<int32_t*>(l + T);
-    refit_top_levels<false>(nodes, T, H, slev, links, top_child, root_out, b, l, sl);
+    refit_top_levels<false>(nodes, T, H, slev, links, top_child, root_out, b, l, sl, wide, wdst);
   }
 }
 
@@ -479,7 +500,7 @@ void Body::top_setup(int k, int l, hipStream_t s) {
   }
 }
 
-void Body::refit(int k, int l, hipStream_t s) {
+void Body::refit(int k, int l, hipStream_t s, bool binary) {
   if (mode == CC_BODY_NONE || m == 0) return;
   if (bottom_up) {                      // same keys, so the same tree; new boxes
     prim_boxes_of(k, l, s);
@@ -489,14 +510,19 @@ void Body::refit(int k, int l, hipStream_t s) {
   Lane& L = lane[l];
   Lane& T = topo(l);
   Slot& S = slot[k];
+  // A body walked through its four-wide form gets only the wide boxes (the
+  // refit reads just the links of the binary records); `binary` asks for
+  // the binary boxes instead (export, or the binary walk).
+  float4* W = (nwide && wide_enabled && !binary) ? S.wide.p : nullptr;
   k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(S.verts.p, leaf_tri_of(k), T.leaf_parent.p,
                                                               S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
                                                               T.chunk_h.p, T.top_pos.p, L.top_child.p, S.root.p,
-                                                              S.leaf0_box.p);
+                                                              S.leaf0_box.p, W, w_dst.p);
   CC_HIP(hipGetLastError());
   if (m > (uint32_t)RC_CHUNK) {
     k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(S.nodes.p, T.top_links.p, T.top_lev.p, L.top_child.p,
-                                                T.top_count.p, T.top_hmax.p, S.root.p, L.top_scratch.p);
+                                                T.top_count.p, T.top_hmax.p, S.root.p, L.top_scratch.p, W,
+                                                w_dst.p);
     CC_HIP(hipGetLastError());
   }
 }

@@ -323,6 +323,17 @@ class Simulator:
     def bvh_refit(self):
         L.check(L.lib().cc_bvh_refit(self._h))
 
+    def set_bvh_width(self, width):
+        """Traversal through the four-wide form (4, default) or the binary
+        tree (2); the contacts are the same."""
+        L.check(L.lib().cc_sim_set_bvh_width(self._h, int(width)))
+
+    def bvh_width(self):
+        """(width in use, wide node count, deferral bound of the tree)."""
+        w, nw, sb = C.c_int(), C.c_uint32(), C.c_uint32()
+        L.check(L.lib().cc_bvh_width(self._h, C.byref(w), C.byref(nw), C.byref(sb)))
+        return w.value, nw.value, sb.value
+
     def bvh_export(self):
         m = self.bvh_size
         mi = max(m - 1, 0)
