@@ -39,6 +39,13 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
     const f3 vel = mk((p.x - q.x) * a.inv_dt, (p.y - q.y) * a.inv_dt, (p.z - q.z) * a.inv_dt);
     f3 F = mk(fmaf(vel.x, a.damp, a.gmx), fmaf(vel.y, a.damp, a.gmy), fmaf(vel.z, a.damp, a.gmz));
     f3 S = mk(0.f, 0.f, 0.f);   // all spring types, in merged-row order
+    // The stiffnesses as wave-uniform values (a is uniform in every caller):
+    // a select between fields of `a` is otherwise compiled into a select of
+    // their addresses and a load, which keeps a caller's local copy of the
+    // arguments in scratch memory.
+    const float ks0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k0)));
+    const float ks1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k1)));
+    const float ks2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k2)));
     // Batches of SPRING_BATCH springs, branch-free: all record loads, then
     // all neighbour gathers, are issued before any use.  Slots past the
     // row end gather the vertex itself and contribute exactly +0 (with a
@@ -63,7 +70,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
       for (int q = 0; q < SPRING_BATCH; ++q) {
         const uint32_t t = rr[q].x >> 30;
         const float rest = __uint_as_float(rr[q].y) * a.rest_scale;
-        const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
+        const float ks = t == 0 ? ks0 : (t == 1 ? ks1 : ks2);
         const f3 p2 = mk(g[q][0].x, g[q][0].y, g[q][1].x), q2 = mk(g[q][1].y, g[q][2].x, g[q][2].y);
         const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
         const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
