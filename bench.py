@@ -87,6 +87,7 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
     time is spent."""
     import oracle as O
     from simulator_amd import _lib as L
+    O.set_threads(nthreads)            # boxes, deformation and refit subtrees; the step takes nthreads
     b = sc.body
     v4 = np.concatenate([b.verts, np.ones((b.verts.shape[0], 1), np.float32)], 1)
     rest = O.vertex_extend(v4, b.tris, b.tris, b.normals, sc.params.body_extend)[:, :3].copy()
@@ -133,6 +134,7 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 200:
             break
+    O.set_threads(1)
     what = {L.CC_BODY_REFIT: "deform + refit + ", L.CC_BODY_REBUILD: "deform + rebuild + "}.get(b.mode, "")
     return {"value": sc.n * steps / el, "unit": "vertex-steps/s", "cores": nthreads, "kind": "port",
             "sample": f"{steps} whole steps ({what}Verlet/spring/collision) of the same {label} workload "
@@ -325,7 +327,14 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, budget_s=args.cpu_budget, label=args.workload)
+        cb = cpu_baseline(sc, budget_s=args.cpu_budget, label=args.workload)
+        # the same on all the host cores this process may use (OpenMP over
+        # vertices, triangles and refit subtrees), at most 16: a GPU's share
+        # of its host
+        cores = min(16, len(os.sched_getaffinity(0)))
+        if cores > 1:
+            cb["all_cores"] = cpu_baseline(sc, budget_s=args.cpu_budget, nthreads=cores, label=args.workload)
+        out["cpu_baseline"] = cb
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib << 20)
     print(json.dumps(out))

@@ -58,6 +58,7 @@ def olib():
             "or_vertex_extend": (None, [u32, V, u32, V, V, V, f32]),
             "or_prim_box": (None, [V, V, f32, V]),
             "or_prim_boxes": (None, [V, u32, V, f32, V]),
+            "or_set_threads": (None, [C.c_int]),
             "or_root_box": (None, [u32, V, V]),
             "or_expand_bits": (u32, [u32]),
             "or_morton3d": (u32, [f32, f32, f32]),
@@ -159,6 +160,12 @@ def vertex_extend(verts4, faces, normal_index, normals3, dist):
 
 
 # ------------------------------------------------------------------ LBVH
+def set_threads(n):
+    """Threads of the oracle's per-item loops (boxes, deformation, refit
+    subtrees); results do not depend on it."""
+    olib().or_set_threads(int(n))
+
+
 def prim_boxes(verts, tris, depth):
     v, t = _c(verts, np.float32), _c(tris, np.uint32)
     m = t.size // 3

@@ -345,7 +345,13 @@ void or_prim_box(const float* verts3, const uint32_t* tri, float depth, float bo
   box_expand_pt(box6, vsub(c, dn));
 }
 
+/* Threads of the loops below that are independent per item (boxes,
+   deformation, refit subtrees); the results do not depend on it. */
+static int g_threads = 1;
+void or_set_threads(int n) { g_threads = n > 1 ? n : 1; }
+
 void or_prim_boxes(const float* verts3, uint32_t m, const uint32_t* tris, float depth, float* boxes6) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (uint32_t i = 0; i < m; ++i) or_prim_box(verts3, tris + 3 * (size_t)i, depth, boxes6 + 6 * (size_t)i);
 }
 
@@ -458,16 +464,17 @@ static const float* child_box(int32_t c, const float* box_leaf, const float* box
   return c < 0 ? box_leaf + 6 * (size_t)(-(c + 1)) : box_int + 6 * (size_t)(c - 1);
 }
 
-void or_refit(uint32_t m, const int32_t* childA, const int32_t* childB, const int32_t* parent_int,
-              const float* box_leaf, float* box_int) {
-  (void)parent_int;
-  if (m < 2) return;
-  /* post-order over the tree from the root (children before parents) */
-  int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * 2 * m);
-  uint8_t* state = (uint8_t*)calloc(m, 1);
+/* Post-order refit (children before parents) of the subtree under x.  The
+   stack holds at most two entries per level of the current path, and a
+   Karras tree over unique 64-bit keys is less than 64 levels deep. */
+enum { REFIT_STACK = 160 };
+static void refit_subtree(int32_t x0, const int32_t* childA, const int32_t* childB, const float* box_leaf,
+                          float* box_int, uint8_t* state) {
+  int32_t stack[REFIT_STACK];
   int sp = 0;
-  stack[sp++] = 0;
+  stack[sp++] = x0;
   while (sp) {
+    assert(sp + 2 <= REFIT_STACK);
     int32_t x = stack[sp - 1];
     if (!state[x]) {
       state[x] = 1;
@@ -481,7 +488,49 @@ void or_refit(uint32_t m, const int32_t* childA, const int32_t* childB, const in
       box_expand(b, child_box(childB[x], box_leaf, box_int));
     }
   }
-  free(stack); free(state);
+}
+
+void or_refit(uint32_t m, const int32_t* childA, const int32_t* childB, const int32_t* parent_int,
+              const float* box_leaf, float* box_int) {
+  (void)parent_int;
+  if (m < 2) return;
+  uint8_t* state = (uint8_t*)calloc(m, 1);
+  /* The internal nodes of the top six levels in breadth-first order; the
+     subtrees under the internal nodes below them are refitted first (in
+     parallel when threads are set: they share no node), then the top nodes
+     in reverse breadth-first order.  fmin / fmax unions are exact, so the
+     boxes do not depend on the order. */
+  enum { TOP_DEPTH = 6 };
+  int32_t top[1 << (TOP_DEPTH + 1)], sub[1 << (TOP_DEPTH + 1)];
+  int ntop = 0, nsub = 0;
+  int32_t lvl[1 << (TOP_DEPTH + 1)], nxt[1 << (TOP_DEPTH + 1)];
+  int nl = 0;
+  lvl[nl++] = 0;
+  for (int d = 0; d <= TOP_DEPTH && nl; ++d) {
+    int nn = 0;
+    for (int q = 0; q < nl; ++q) {
+      const int32_t x = lvl[q];
+      if (d == TOP_DEPTH) {
+        sub[nsub++] = x;
+        continue;
+      }
+      top[ntop++] = x;
+      if (childA[x] > 0) nxt[nn++] = childA[x] - 1;
+      if (childB[x] > 0) nxt[nn++] = childB[x] - 1;
+    }
+    memcpy(lvl, nxt, sizeof(int32_t) * (size_t)nn);
+    nl = nn;
+  }
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(dynamic, 1)
+  for (int q = 0; q < nsub; ++q) refit_subtree(sub[q], childA, childB, box_leaf, box_int, state);
+  for (int q = ntop - 1; q >= 0; --q) {
+    const int32_t x = top[q];
+    float* b = box_int + 6 * (size_t)x;
+    box_sentinel(b);
+    box_expand(b, child_box(childA[x], box_leaf, box_int));
+    box_expand(b, child_box(childB[x], box_leaf, box_int));
+  }
+  free(state);
 }
 
 uint32_t or_bvh_build(const float* verts3, uint32_t m, const uint32_t* tris, float depth,
@@ -1032,9 +1081,14 @@ void or_collide_points(uint32_t n, const float* pts, const or_bvh* bvh,
 void or_body_deform(uint32_t nv, const float* rest3, uint32_t ncaps, const uint32_t* cap_vstart,
                     const float* cap_p0, const float* cap_p1, const float* breath,
                     const float R[9], const float T[3], float* out3) {
-  uint32_t k = 0;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (uint32_t v = 0; v < nv; ++v) {
-    while (k + 1 < ncaps && v >= cap_vstart[k + 1]) ++k;
+    uint32_t lo = 0, hi = ncaps - 1;     /* k = the last capsule with cap_vstart[k] <= v */
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (cap_vstart[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t k = lo;
     v3f X = f3(rest3 + 3 * (size_t)v);
     v3f p0 = f3(cap_p0 + 3 * (size_t)k), p1 = f3(cap_p1 + 3 * (size_t)k);
     v3f d = vsub(p1, p0);

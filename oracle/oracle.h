@@ -108,6 +108,9 @@ void or_vertex_extend(uint32_t nv, float* verts4, uint32_t nf, const uint32_t* f
 
 /* --- LBVH, FP32 exact order --- */
 void or_prim_box(const float* verts3, const uint32_t* tri, float depth, float box6[6]);
+/* Threads for the per-item loops (boxes, deformation, refit subtrees);
+   results do not depend on it.  Default 1. */
+void or_set_threads(int n);
 void or_prim_boxes(const float* verts3, uint32_t m, const uint32_t* tris, float depth, float* boxes6);
 void or_root_box(uint32_t m, const float* boxes6, float root6[6]);
 uint32_t or_expand_bits(uint32_t v);
