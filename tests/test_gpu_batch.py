@@ -194,3 +194,35 @@ def test_batch_edge_instances(gpu):
     for s in sims:
         s.close()
     b.close()
+
+
+def test_batch_sort_tile_sizes(gpu):
+    """Bodies around the per-body sort's tile of 8192 keys: 8192 triangles
+    (one full tile), 8193 and 16391 (a last tile of one key), 21000 (three
+    tiles), rebuilt every frame; every tree equals the oracle's build and the
+    single simulator's, bit for bit."""
+    from simulator_amd import scene as S
+    v, t, nrm, caps = S.capsule_body(12, 0.04, 0.15, *S.CAPSULE_BOX, 96, 24, 24, 3)
+    assert t.shape[0] >= 21000
+    scs = []
+    for m in (8192, 8193, 16391, 21000):
+        body = S.Body(v, np.ascontiguousarray(t[:m]), nrm, L.CC_BODY_REBUILD, True, caps, S._motion(5))
+        scs.append(S.grid_scene("tiles", 24, 1e-3, 40, body, box_depth=0.002))
+    b = B.Batch(scs)
+    sims = [Simulator.from_scene(sc) for sc in scs]
+    for frame in (0, 1, 17):
+        if frame:
+            step = frame - b.frame
+            b.simulate(step)
+            for s in sims:
+                s.simulate(step)
+            b.synchronize()
+        for k, sc in enumerate(scs):
+            say(f"[batch] sort tiles, frame {frame}, {sc.body.tris.shape[0]} triangles")
+            pose, tree = b.body_export(k)
+            np.testing.assert_array_equal(pose, sims[k].body_vertices())
+            assert_bvh_equal(tree, O.bvh_build(pose, sc.body.tris, sc.params.box_depth))
+            assert_bvh_equal(tree, sims[k].bvh_export())
+    for s in sims:
+        s.close()
+    b.close()
