@@ -12,6 +12,12 @@ constexpr int SPRING_BATCH = 8;
 constexpr uint32_t PIN_BIT = 0x80000000u;
 constexpr uint32_t ROW_MASK = 0x7fffffffu;
 constexpr uint32_t END_MASK = 0x3fffffffu;
+// Fixed-width spring rows: a cloth whose vertices have at most ELL_W springs
+// (the grid cloths: 4 structural, 4 shear, 4 bend) also keeps its merged
+// records as ELL_W per vertex (record k of vertex i at i * ELL_W + k), empty
+// slots {i | PAD_TYPE, 0}, so a vertex's records load with its own data.
+constexpr int ELL_W = 12;
+constexpr uint32_t PAD_TYPE = 3u << 30;
 
 struct StepArgs {
   float inv_dt, dt2, inv_mass;
@@ -34,6 +40,8 @@ struct Sim {
   DBuf<uint32_t> row;
   DBuf<int32_t> spill;      // [STACK_SPILL * n] traversal stack spill of the step kernel
   DBuf<uint2> rec;
+  DBuf<uint2> ell;          // [ELL_W * n] the same records, fixed width (when use_ell)
+  bool use_ell = false;
   std::vector<uint32_t> h_row;
   int cur = 0;              // state[cur] holds frame `frame`
   uint64_t frame = 0;

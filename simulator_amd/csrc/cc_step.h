@@ -7,12 +7,40 @@
 
 namespace cc {
 
+// One spring's term of compute_spring_force (verlet.cu:52-88) added to S:
+// record rr {end | type << 30, rest}, neighbour state g (x_cur, x_prev as
+// three float2).  An empty slot (real false) gathers the vertex itself:
+// dP = (eps, eps, eps) > 0 and sc = +0, so the products are +0 and S (never
+// -0) is unchanged.
+__device__ inline void spring_term(f3 p, f3 vel, const float2* g, uint2 rr, bool real, const StepArgs& a,
+                                   float ks0, float ks1, float ks2, f3& S) {
+  const uint32_t t = rr.x >> 30;
+  const float rest = __uint_as_float(rr.y) * a.rest_scale;
+  const float ks = t == 0 ? ks0 : (t == 1 ? ks1 : ks2);
+  const f3 p2 = mk(g[0].x, g[0].y, g[1].x), q2 = mk(g[1].y, g[2].x, g[2].y);
+  const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
+  const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
+                   fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
+  const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
+  const float rinv = rsqrtf(L2);
+  const float L = L2 * rinv;
+  const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
+  const float sc = real ? fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv : 0.0f;
+  S.x = fmaf(dP.x, sc, S.x);
+  S.y = fmaf(dP.y, sc, S.y);
+  S.z = fmaf(dP.z, sc, S.z);
+}
+
 // Vertex i from state `sin_` to `sout` (24 B {x_cur, x_prev} per vertex),
 // updating its contact normal in place.  Spring forces use FMA and rsqrt
 // (positions are checked against the FP64 oracle by tolerance); the
 // collision part is exact order so the contact set is bit-exact for the
 // pre-projection positions.  verlet.cu:129-161, 52-88, 24-50.
-template <bool DEBUG>
+// ELL: `rec` holds ELL_W records per vertex (see ELL_W), loaded with the
+// vertex's own records; otherwise `rec` is the merged CSR and `row` its
+// offsets, and the records load once the offsets have arrived.  Both sum the
+// same springs in the same order.
+template <bool DEBUG, bool ELL>
 __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, float* __restrict__ sout,
                                    float* __restrict__ contact, const uint32_t* __restrict__ row,
                                    const uint2* __restrict__ rec, const StepArgs& a, const BvhView& bvh,
@@ -21,7 +49,12 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
   // The vertex's own records are loaded together and waited for once: the
   // compiler otherwise issues each load at its first use, one memory round
   // trip after another.
-  const uint32_t r0raw = row[i], r1raw = row[i + 1];
+  uint2 er[ELL ? ELL_W : 1];
+  if constexpr (ELL) {
+#pragma unroll
+    for (int k = 0; k < ELL_W; ++k) er[k] = rec[(size_t)i * ELL_W + k];
+  }
+  const uint32_t r0raw = row[i], r1raw = ELL ? 0u : row[i + 1];
   const float2* s2 = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)i);
   const float2 e0 = s2[0], e1 = s2[1], e2 = s2[2];
   const f3 c = ld3(contact + 3 * (size_t)i);
@@ -46,45 +79,43 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
     const float ks0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k0)));
     const float ks1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k1)));
     const float ks2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k2)));
-    // Batches of SPRING_BATCH springs, branch-free: all record loads, then
-    // all neighbour gathers, are issued before any use.  Slots past the
-    // row end gather the vertex itself and contribute exactly +0 (with a
-    // condition per slot the compiler sinks each gather into its branch,
-    // one memory round trip per spring).
-    for (uint32_t k0 = r0; k0 < r1; k0 += SPRING_BATCH) {
-      uint2 rr[SPRING_BATCH];
-      float2 g[SPRING_BATCH][3];
+    if constexpr (ELL) {
+      // all neighbour gathers issued before any use
+      float2 g[ELL_W][3];
 #pragma unroll
-      for (int q = 0; q < SPRING_BATCH; ++q) {
-        rr[q] = rec[min(k0 + (uint32_t)q, r1 - 1)];
-        if (k0 + q >= r1) rr[q] = make_uint2(i, 0u);
+      for (int k = 0; k < ELL_W; ++k) {
+        const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(er[k].x & END_MASK));
+        g[k][0] = sj[0];
+        g[k][1] = sj[1];
+        g[k][2] = sj[2];
       }
 #pragma unroll
-      for (int q = 0; q < SPRING_BATCH; ++q) {
-        const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(rr[q].x & END_MASK));
-        g[q][0] = sj[0];
-        g[q][1] = sj[1];
-        g[q][2] = sj[2];
-      }
+      for (int k = 0; k < ELL_W; ++k)
+        spring_term(p, vel, g[k], er[k], (er[k].x & PAD_TYPE) != PAD_TYPE, a, ks0, ks1, ks2, S);
+    } else {
+      // Batches of SPRING_BATCH springs, branch-free: all record loads, then
+      // all neighbour gathers, are issued before any use.  Slots past the
+      // row end gather the vertex itself (with a condition per slot the
+      // compiler sinks each gather into its branch, one memory round trip
+      // per spring).
+      for (uint32_t k0 = r0; k0 < r1; k0 += SPRING_BATCH) {
+        uint2 rr[SPRING_BATCH];
+        float2 g[SPRING_BATCH][3];
 #pragma unroll
-      for (int q = 0; q < SPRING_BATCH; ++q) {
-        const uint32_t t = rr[q].x >> 30;
-        const float rest = __uint_as_float(rr[q].y) * a.rest_scale;
-        const float ks = t == 0 ? ks0 : (t == 1 ? ks1 : ks2);
-        const f3 p2 = mk(g[q][0].x, g[q][0].y, g[q][1].x), q2 = mk(g[q][1].y, g[q][2].x, g[q][2].y);
-        const f3 dP = mk((p.x - p2.x) + a.eps, (p.y - p2.y) + a.eps, (p.z - p2.z) + a.eps);
-        const f3 dV = mk(fmaf(-(p2.x - q2.x), a.inv_dt, vel.x), fmaf(-(p2.y - q2.y), a.inv_dt, vel.y),
-                         fmaf(-(p2.z - q2.z), a.inv_dt, vel.z));
-        const float L2 = fmaf(dP.z, dP.z, fmaf(dP.y, dP.y, dP.x * dP.x));
-        const float rinv = rsqrtf(L2);
-        const float L = L2 * rinv;
-        const float vd = fmaf(dV.z, dP.z, fmaf(dV.y, dP.y, dV.x * dP.x));
-        // an empty slot: dP = (eps, eps, eps) > 0 and sc = +0, so the
-        // products are +0 and S (never -0) is unchanged
-        const float sc = (k0 + q < r1) ? fmaf(-ks, L - rest, a.kd * vd * rinv) * rinv : 0.0f;
-        S.x = fmaf(dP.x, sc, S.x);
-        S.y = fmaf(dP.y, sc, S.y);
-        S.z = fmaf(dP.z, sc, S.z);
+        for (int q = 0; q < SPRING_BATCH; ++q) {
+          rr[q] = rec[min(k0 + (uint32_t)q, r1 - 1)];
+          if (k0 + q >= r1) rr[q] = make_uint2(i, 0u);
+        }
+#pragma unroll
+        for (int q = 0; q < SPRING_BATCH; ++q) {
+          const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(rr[q].x & END_MASK));
+          g[q][0] = sj[0];
+          g[q][1] = sj[1];
+          g[q][2] = sj[2];
+        }
+#pragma unroll
+        for (int q = 0; q < SPRING_BATCH; ++q)
+          spring_term(p, vel, g[q], rr[q], k0 + q < r1, a, ks0, ks1, ks2, S);
       }
     }
     F = F + S;

@@ -22,7 +22,7 @@ namespace cc {
 // One thread per vertex.  Spring forces use FMA and rsqrt (positions are
 // checked against the FP64 oracle by tolerance); the collision part is exact
 // order so the contact set is bit-exact for the pre-projection positions.
-template <bool DEBUG, bool COUNT>
+template <bool DEBUG, bool COUNT, bool ELL>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
                                                 float* __restrict__ contact, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
@@ -33,8 +33,8 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    step_vertex<DEBUG>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre, dbg_hit,
-                       dbg_in);
+    step_vertex<DEBUG, ELL>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
+                            dbg_hit, dbg_in);
   }
   if (COUNT) {
     // block sum into this block's own counter pair (no shared atomics)
@@ -125,6 +125,19 @@ StepArgs step_args(const cc_params& prm, uint32_t ntypes) {
 }
 
 // ---------------------------------------------------------- Sim launches
+template <bool DEBUG, bool COUNT>
+static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, float* contact_io,
+                            const StepArgs& a, const BvhView& v, float* dbg_pre, int32_t* dbg_hit, uint8_t* dbg_in,
+                            unsigned long long* ctr) {
+  if (S.use_ell)
+    k_verlet<DEBUG, COUNT, true><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
+                                                   dbg_pre, dbg_hit, dbg_in, ctr);
+  else
+    k_verlet<DEBUG, COUNT, false><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
+                                                    dbg_pre, dbg_hit, dbg_in, ctr);
+  CC_HIP(hipGetLastError());
+}
+
 void Sim::launch_verlet(int k, hipStream_t s) {
   const float* in = state[cur].p;
   float* out = state[cur ^ 1].p;
@@ -132,24 +145,18 @@ void Sim::launch_verlet(int k, hipStream_t s) {
   const unsigned g = grid_for(n, 256);
   const StepArgs a = args();
   if (debug && counting)
-    k_verlet<true, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, a, v, n, spill.p, dbg_pre.p, dbg_hit.p,
-                                           dbg_in.p, ctr.p);
+    launch_k_verlet<true, true>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else if (debug)
-    k_verlet<true, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, a, v, n, spill.p, dbg_pre.p, dbg_hit.p,
-                                            dbg_in.p, ctr.p);
+    launch_k_verlet<true, false>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else if (counting)
-    k_verlet<false, true><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, a, v, n, spill.p, dbg_pre.p, dbg_hit.p,
-                                            dbg_in.p, ctr.p);
+    launch_k_verlet<false, true>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else
-    k_verlet<false, false><<<g, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, a, v, n, spill.p, dbg_pre.p,
-                                             dbg_hit.p, dbg_in.p, ctr.p);
-  CC_HIP(hipGetLastError());
+    launch_k_verlet<false, false>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
 }
 
 void Sim::launch_step(const float* in, float* out, float* contact_io, const StepArgs& a, int k, hipStream_t s) {
-  k_verlet<false, false><<<grid_for(n, 256), 256, 0, s>>>(in, out, contact_io, row.p, rec.p, a, body.view(k), n,
-                                                          spill.p, nullptr, nullptr, nullptr, nullptr);
-  CC_HIP(hipGetLastError());
+  launch_k_verlet<false, false>(*this, grid_for(n, 256), s, in, out, contact_io, a, body.view(k), nullptr, nullptr,
+                                nullptr, nullptr);
 }
 
 }  // namespace cc
@@ -275,6 +282,22 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         packed[6 * (size_t)i + c] = x3[3 * (size_t)i + c];
         packed[6 * (size_t)i + 3 + c] = x3[3 * (size_t)i + c];
       }
+    // fixed-width copy of the records when every vertex has at most ELL_W
+    // springs (CC_SPRINGS_ELL=0 keeps the merged rows only)
+    uint32_t max_deg = 0;
+    for (uint32_t i = 0; i < S.n; ++i) max_deg = std::max(max_deg, S.h_row[i + 1] - S.h_row[i]);
+    bool ell = max_deg <= (uint32_t)ELL_W;
+    if (const char* e = std::getenv("CC_SPRINGS_ELL")) ell = ell && std::atoi(e) != 0;
+    if (ell) {
+      std::vector<uint2> er((size_t)S.n * ELL_W);
+      for (uint32_t i = 0; i < S.n; ++i)
+        for (uint32_t k = 0; k < (uint32_t)ELL_W; ++k) {
+          const uint32_t e = S.h_row[i] + k;
+          er[(size_t)i * ELL_W + k] = e < S.h_row[i + 1] ? rec[e] : make_uint2(i | PAD_TYPE, 0u);
+        }
+      S.ell.upload(er.data(), er.size(), s);
+      S.use_ell = true;
+    }
     S.spill.alloc((size_t)STACK_SPILL * S.n);
     S.state[0].upload(packed.data(), packed.size(), s);
     S.state[1].alloc(packed.size());
