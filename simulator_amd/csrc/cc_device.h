@@ -87,6 +87,27 @@ __device__ inline bool in_box(f3 p, float lx, float ly, float lz, float hx, floa
   return (p.x >= lx) & (p.x <= hx) & (p.y >= ly) & (p.y <= hy) & (p.z >= lz) & (p.z <= hz);
 }
 
+// The root record, loaded by the caller with its other first loads (the
+// same address for every lane), so the walk starts without a round trip.
+struct WideRoot {
+  float4 q[8];
+};
+
+// The root record of the wide form for a walk (all lanes read the same
+// record); unused when the view has no wide form.
+template <bool WIDE = true>
+__device__ inline WideRoot load_wide_root(const BvhView& b) {
+  WideRoot r;
+  if (WIDE && b.wide && b.m >= 2) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r.q[k] = b.wide[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r.q[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  return r;
+}
+
 // The same first hit through the four-wide nodes.  A binary node X with
 // children A, B considers, in this order: A if it is a leaf, B if it is a
 // leaf, then the subtree of A, then the subtree of B (trav_step).  A wide
@@ -102,37 +123,46 @@ __device__ inline bool in_box(f3 p, float lx, float ly, float lz, float hx, floa
 // The set-up keeps the wide form only when no root path defers more than
 // 64 entries (sum of entries - 1 over the path), so the stack cannot
 // overflow.
-__device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill) {
+// One visit of a wide record q[0..7] (entry e = q[2e], q[2e + 1]): >= 0 the
+// hit leaf, WIDE_NEXT with the next record in `node`, WIDE_MISS.
+enum : int { WIDE_NEXT = -1, WIDE_MISS = -2 };
+__device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4, float4 q5,
+                                     float4 q6, float4 q7, TravStack& st, int32_t& node) {
+  uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
+                  (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
+                  (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
+                  (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
+  const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
+                c3 = __float_as_int(q7.z);
+  if (mask == 0) {
+    if (!st.pop(node)) return WIDE_MISS;
+    return node < 0 ? ~node : WIDE_NEXT;
+  }
+  const uint32_t first = __builtin_ctz(mask);
+  const int32_t c = first == 0 ? c0 : first == 1 ? c1 : first == 2 ? c2 : c3;
+  if (c < 0) return ~c;
+  mask &= mask - 1;
+  if ((mask & 8u) && !st.push(c3)) return WIDE_MISS;
+  if ((mask & 4u) && !st.push(c2)) return WIDE_MISS;
+  if ((mask & 2u) && !st.push(c1)) return WIDE_MISS;
+  node = c;
+  return WIDE_NEXT;
+}
+
+__device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill, const WideRoot& rt) {
   TravStack st;
   st.spill = spill;
   int32_t node = 0;
-  for (uint32_t guard = 0; guard < b.m; ++guard) {
-    // entry e = q[2e], q[2e + 1]: min xyz, max x | max yz, link
+  ++visits;
+  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], st, node);
+  for (uint32_t guard = 0; r == WIDE_NEXT && guard < b.m; ++guard) {
     const float4* nd = b.wide + 8 * (size_t)node;
     const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6],
                  q7 = nd[7];
     ++visits;
-    uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
-                    (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
-                    (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
-                    (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
-    const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
-                  c3 = __float_as_int(q7.z);
-    if (mask == 0) {
-      if (!st.pop(node)) return -1;
-      if (node < 0) return ~node;
-      continue;
-    }
-    const uint32_t first = __builtin_ctz(mask);
-    const int32_t c = first == 0 ? c0 : first == 1 ? c1 : first == 2 ? c2 : c3;
-    if (c < 0) return ~c;
-    mask &= mask - 1;
-    if ((mask & 8u) && !st.push(c3)) return -1;
-    if ((mask & 4u) && !st.push(c2)) return -1;
-    if ((mask & 2u) && !st.push(c1)) return -1;
-    node = c;
+    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, st, node);
   }
-  return -1;
+  return r >= 0 ? r : -1;
 }
 
 // First-hit traversal (D_BVH::intersect, cudaBRT.cu:239-285): one node
@@ -141,10 +171,14 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
 // Karras tree over unique 64-bit keys is below 64, so 64 entries cannot
 // overflow.  Returns the sorted leaf index, or -1.  *visits counts internal
 // nodes processed (wide nodes when the view has them).
-__device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill) {
+// WIDE false: the caller's views never have the wide form (batched bodies),
+// and the wide walk is left out of its kernel.
+template <bool WIDE = true>
+__device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill,
+                                       const WideRoot& rt) {
   if (b.m == 0) return -1;
   if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
-  if (b.wide) return wide_first_hit(b, p, visits, spill);
+  if (WIDE && b.wide) return wide_first_hit(b, p, visits, spill, rt);
   TravStack st;
   st.spill = spill;
   int32_t node = 0;
@@ -175,9 +209,10 @@ __device__ inline bool plane_test(const float* verts, uint4 t, f3 p, float& d, f
 // collision_response_projection (verlet.cu:24-50) on a position p: when the
 // first-hit leaf's plane test says inside, p += |d| n and the contact normal
 // is n, else the contact normal is 0.  Returns true when projected.
+template <bool WIDE = true>
 __device__ inline bool collide_project(const BvhView& b, f3& p, f3& n_out, int32_t& hit,
-                                       uint32_t& visits, Spill spill) {
-  hit = bvh_first_hit(b, p, visits, spill);
+                                       uint32_t& visits, Spill spill, const WideRoot& rt) {
+  hit = bvh_first_hit<WIDE>(b, p, visits, spill, rt);
   n_out = mk(0.0f, 0.0f, 0.0f);
   if (hit < 0) return false;
   float d;

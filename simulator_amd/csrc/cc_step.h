@@ -40,7 +40,9 @@ __device__ inline void spring_term(f3 p, f3 vel, const float2* g, uint2 rr, bool
 // vertex's own records; otherwise `rec` is the merged CSR and `row` its
 // offsets, and the records load once the offsets have arrived.  Both sum the
 // same springs in the same order.
-template <bool DEBUG, bool ELL>
+// WIDE: the view may have the four-wide form (single bodies; the batched
+// bodies never do).
+template <bool DEBUG, bool ELL, bool WIDE>
 __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, float* __restrict__ sout,
                                    float* __restrict__ contact, const uint32_t* __restrict__ row,
                                    const uint2* __restrict__ rec, const StepArgs& a, const BvhView& bvh,
@@ -54,6 +56,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
 #pragma unroll
     for (int k = 0; k < ELL_W; ++k) er[k] = rec[(size_t)i * ELL_W + k];
   }
+  const WideRoot root = load_wide_root<WIDE>(bvh);
   const uint32_t r0raw = row[i], r1raw = ELL ? 0u : row[i + 1];
   const float2* s2 = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)i);
   const float2 e0 = s2[0], e1 = s2[1], e2 = s2[2];
@@ -128,7 +131,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
             p.z + fmaf(F.z * a.inv_mass, a.dt2, p.z - q.z));
     nq = p;
     if (DEBUG) st3(dbg_pre + 3 * (size_t)i, np);
-    inside = collide_project(bvh, np, nc, hit, visits, spill);
+    inside = collide_project<WIDE>(bvh, np, nc, hit, visits, spill, root);
     if (inside) nq = np;
   }
   if (DEBUG) {

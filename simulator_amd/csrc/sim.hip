@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    step_vertex<DEBUG, ELL>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
+    step_vertex<DEBUG, ELL, true>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
                             dbg_hit, dbg_in);
   }
   if (COUNT) {
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float
   f3 p = ld3(pts + 3 * (size_t)i), nc;
   int32_t h;
   uint32_t visits = 0;
-  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n});
+  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n}, load_wide_root(bvh));
   hit[i] = h;
   ins[i] = in ? 1 : 0;
   st3(out_pos + 3 * (size_t)i, p);
