@@ -13,10 +13,12 @@ constexpr uint32_t PIN_BIT = 0x80000000u;
 constexpr uint32_t ROW_MASK = 0x7fffffffu;
 constexpr uint32_t END_MASK = 0x3fffffffu;
 // Fixed-width spring rows: a cloth whose vertices have at most ELL_W springs
-// (the grid cloths: 4 structural, 4 shear, 4 bend) also keeps its merged
-// records as ELL_W per vertex (record k of vertex i at i * ELL_W + k), empty
-// slots {i | PAD_TYPE, 0}, so a vertex's records load with its own data.
+// (the grid cloths: 4 structural, 4 shear, 4 bend), or at most ELL_W2 (OBJ
+// meshes: their 1-ring and 2-ring springs), also keeps its merged records as
+// W per vertex (record k of vertex i at i * W + k), empty slots
+// {i | PAD_TYPE, 0}, so a vertex's records load with its own data.
 constexpr int ELL_W = 12;
+constexpr int ELL_W2 = 24;
 constexpr uint32_t PAD_TYPE = 3u << 30;
 
 struct StepArgs {
@@ -40,8 +42,8 @@ struct Sim {
   DBuf<uint32_t> row;
   DBuf<int32_t> spill;      // [STACK_SPILL * n] traversal stack spill of the step kernel
   DBuf<uint2> rec;
-  DBuf<uint2> ell;          // [ELL_W * n] the same records, fixed width (when use_ell)
-  bool use_ell = false;
+  DBuf<uint2> ell;          // [ell_w * n] the same records, fixed width (when ell_w > 0)
+  int ell_w = 0;            // 0 (merged rows only), ELL_W or ELL_W2
   std::vector<uint32_t> h_row;
   int cur = 0;              // state[cur] holds frame `frame`
   uint64_t frame = 0;

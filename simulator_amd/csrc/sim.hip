@@ -22,7 +22,7 @@ namespace cc {
 // One thread per vertex.  Spring forces use FMA and rsqrt (positions are
 // checked against the FP64 oracle by tolerance); the collision part is exact
 // order so the contact set is bit-exact for the pre-projection positions.
-template <bool DEBUG, bool COUNT, bool ELL>
+template <bool DEBUG, bool COUNT, int EW>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
                                                 float* __restrict__ contact, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    step_vertex<DEBUG, ELL, true>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
+    step_vertex<DEBUG, EW, true>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
                             dbg_hit, dbg_in);
   }
   if (COUNT) {
@@ -129,12 +129,15 @@ template <bool DEBUG, bool COUNT>
 static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, float* contact_io,
                             const StepArgs& a, const BvhView& v, float* dbg_pre, int32_t* dbg_hit, uint8_t* dbg_in,
                             unsigned long long* ctr) {
-  if (S.use_ell)
-    k_verlet<DEBUG, COUNT, true><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
-                                                   dbg_pre, dbg_hit, dbg_in, ctr);
-  else
-    k_verlet<DEBUG, COUNT, false><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
+  if (S.ell_w == ELL_W)
+    k_verlet<DEBUG, COUNT, ELL_W><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
                                                     dbg_pre, dbg_hit, dbg_in, ctr);
+  else if (S.ell_w == ELL_W2)
+    k_verlet<DEBUG, COUNT, ELL_W2><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
+                                                     dbg_pre, dbg_hit, dbg_in, ctr);
+  else
+    k_verlet<DEBUG, COUNT, 0><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
+                                                dbg_pre, dbg_hit, dbg_in, ctr);
   CC_HIP(hipGetLastError());
 }
 
@@ -283,20 +286,24 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         packed[6 * (size_t)i + 3 + c] = x3[3 * (size_t)i + c];
       }
     // fixed-width copy of the records when every vertex has at most ELL_W
-    // springs (CC_SPRINGS_ELL=0 keeps the merged rows only)
+    // (or ELL_W2) springs (CC_SPRINGS_ELL=0 keeps the merged rows only)
     uint32_t max_deg = 0;
     for (uint32_t i = 0; i < S.n; ++i) max_deg = std::max(max_deg, S.h_row[i + 1] - S.h_row[i]);
-    bool ell = max_deg <= (uint32_t)ELL_W;
-    if (const char* e = std::getenv("CC_SPRINGS_ELL")) ell = ell && std::atoi(e) != 0;
-    if (ell) {
-      std::vector<uint2> er((size_t)S.n * ELL_W);
+    // the 24-wide kernel runs two waves per SIMD: only for cloths whose
+    // waves all fit at once on the chip (256 CUs x 8 waves x 64 vertices)
+    uint32_t w = max_deg <= (uint32_t)ELL_W ? ELL_W
+                 : (max_deg <= (uint32_t)ELL_W2 && S.n <= 256u * 8u * 64u) ? ELL_W2 : 0;
+    if (const char* e = std::getenv("CC_SPRINGS_ELL"))
+      if (std::atoi(e) == 0) w = 0;
+    if (w) {
+      std::vector<uint2> er((size_t)S.n * w);
       for (uint32_t i = 0; i < S.n; ++i)
-        for (uint32_t k = 0; k < (uint32_t)ELL_W; ++k) {
+        for (uint32_t k = 0; k < w; ++k) {
           const uint32_t e = S.h_row[i] + k;
-          er[(size_t)i * ELL_W + k] = e < S.h_row[i + 1] ? rec[e] : make_uint2(i | PAD_TYPE, 0u);
+          er[(size_t)i * w + k] = e < S.h_row[i + 1] ? rec[e] : make_uint2(i | PAD_TYPE, 0u);
         }
       S.ell.upload(er.data(), er.size(), s);
-      S.use_ell = true;
+      S.ell_w = (int)w;
     }
     S.spill.alloc((size_t)STACK_SPILL * S.n);
     S.state[0].upload(packed.data(), packed.size(), s);
