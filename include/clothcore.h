@@ -83,6 +83,19 @@ int cc_device_count(int* count);
    face).  Rest length = distance at set-up. */
 int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
                          cc_csr* out_structure, cc_csr* out_bend);
+/* Springs::Springs for a SINGLE_LAYER_BOUNDARY mesh (spring.cpp:82-91,
+   98-209, kdtree.cpp): the vertex groups, in pairs {cloth piece, its
+   boundary}, add boundary springs to the structural rows before the CSR is
+   built: each boundary vertex to its nearest piece vertex (both directions),
+   and, for boundary vertices whose nearest vertex on another boundary lies
+   at 0 < d < 50 x the mean of the first 100 such ties, their piece vertices
+   (one direction, sorted pairs).  Nearest neighbours follow the reference's
+   kd-tree, ties included.  Rejected (the reference reads out of range):
+   an odd group count, groups beyond nv, an empty piece, fewer than 100
+   boundary vertices, one boundary alone.  cc_springs_from_mesh is the
+   SINGLE_LAYER_NOB case the reference's scene uses (main.cpp:24). */
+int cc_springs_from_mesh_groups(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                                const uint32_t* group_counts, uint32_t ngroups, cc_csr* out_s, cc_csr* out_b);
 
 /* Grid cloth springs: type 0 structural (4-neighbour), type 1 shear (both
    diagonals), type 2 bend (skip-one).  Vertex (i, j) is index j*nx + i;
@@ -149,6 +162,11 @@ typedef struct cc_obj {
   uint32_t* vidx;     /* [3 nf] vertex index per corner  */
   uint32_t* tidx;     /* [3 nf] texture index per corner */
   uint32_t* nidx;     /* [3 nf] normal index per corner  */
+  /* vertex groups (ObjLoader::vertex_object, ObjLoader.cpp:34-41): a line
+     whose second token is "object" opens a group, one whose third token is
+     "vertices" sets the last group's vertex count (atoi of the second) */
+  uint32_t ngroups;
+  uint32_t* group_counts;   /* [ngroups] */
 } cc_obj;
 int cc_obj_read(const char* path, cc_obj* out);
 void cc_obj_free(cc_obj* obj);

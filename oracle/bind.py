@@ -55,6 +55,9 @@ def olib():
             "or_grid_springs": (None, [u32, u32, V, V, V, V, V, V, V]),
             "or_icosphere": (u32, [u32, f64, V, V, V, V]),
             "or_springs_from_mesh": (C.c_int, [u32, V, u32, V, V, V, V, V, V, V]),
+            "or_springs_from_mesh_ex": (C.c_int, [u32, V, u32, V, u32, V, u32, V, V, V, V, V, V, V]),
+            "or_kd_nearest": (C.c_int32, [u32, V, u32, V, V]),
+            "or_boundary_springs": (C.c_int32, [u32, V, u32, V, V, V, V]),
             "or_vertex_extend": (None, [u32, V, u32, V, V, V, f32]),
             "or_prim_box": (None, [V, V, f32, V]),
             "or_prim_boxes": (None, [V, u32, V, f32, V]),
@@ -135,18 +138,56 @@ def icosphere(level, radius, center):
     return v, t
 
 
-def springs_from_mesh(verts4, faces):
-    """Springs::create_neigh + CSR_spring (spring.cpp:286-391)."""
+BOUNDARY_ERRORS = {-1: "odd group count", -2: "groups past the vertex count", -3: "an empty piece",
+                   -4: "fewer than 100 boundary vertices", -5: "a boundary with no other boundary"}
+
+
+def kd_nearest(pts, queries):
+    """Index in pts of each query's nearest point, as the reference's
+    kd-tree (kdtree.cpp) finds it, ties included."""
+    p, q = _c(pts, np.float32), _c(queries, np.float32)
+    out = np.zeros(q.shape[0], np.uint32)
+    if olib().or_kd_nearest(p.shape[0], _p(p), q.shape[0], _p(q), _p(out)) < 0:
+        raise ValueError("kd_nearest: no points")
+    return out
+
+
+def boundary_springs(verts4, vertex_object):
+    """(cloth-boundary pairs (k, 2), boundary set pairs (j, 2)) of
+    Springs::get_cloth_boundary_spring / get_boundary_boundary_spring
+    (spring.cpp:98-209); ValueError for the inputs the reference reads out
+    of range."""
+    v = _c(verts4, np.float32)
+    g = _c(vertex_object, np.uint32)
+    nv = v.shape[0]
+    cb = np.zeros((nv + 1, 2), np.uint32)
+    bd = np.zeros((nv + 1, 2), np.uint32)
+    nbd = C.c_uint32()
+    r = olib().or_boundary_springs(nv, _p(v), g.size, _p(g), _p(cb), _p(bd), C.byref(nbd))
+    if r < 0:
+        raise ValueError("boundary springs: " + BOUNDARY_ERRORS[r])
+    return cb[:r].copy(), bd[:nbd.value].copy()
+
+
+def springs_from_mesh(verts4, faces, vertex_object=None):
+    """Springs::Springs + create_neigh + CSR_spring (spring.cpp:82-391):
+    SINGLE_LAYER_NOB, or SINGLE_LAYER_BOUNDARY given the group counts."""
     v = _c(verts4, np.float32)
     f = _c(faces, np.uint32)
     nv, nf = v.shape[0], f.size // 3
+    cbp = np.zeros((0, 2), np.uint32)
+    bdp = np.zeros((0, 2), np.uint32)
+    if vertex_object is not None:
+        cbp, bdp = boundary_springs(v, vertex_object)
+    cbp, bdp = _c(cbp, np.uint32), _c(bdp, np.uint32)
     ns, nb = C.c_uint32(), C.c_uint32()
     L = olib()
-    L.or_springs_from_mesh(nv, _p(v), nf, _p(f), None, None, C.byref(ns), None, None, C.byref(nb))
+    L.or_springs_from_mesh_ex(nv, _p(v), nf, _p(f), cbp.shape[0], _p(cbp), bdp.shape[0], _p(bdp), None, None,
+                              C.byref(ns), None, None, C.byref(nb))
     rs, rb = np.zeros(nv + 1, np.uint32), np.zeros(nv + 1, np.uint32)
     cs, cb = np.zeros(ns.value, SPRING_DTYPE), np.zeros(nb.value, SPRING_DTYPE)
-    L.or_springs_from_mesh(nv, _p(v), nf, _p(f), _p(rs), _p(cs), C.byref(ns), _p(rb), _p(cb),
-                           C.byref(nb))
+    L.or_springs_from_mesh_ex(nv, _p(v), nf, _p(f), cbp.shape[0], _p(cbp), bdp.shape[0], _p(bdp), _p(rs),
+                              _p(cs), C.byref(ns), _p(rb), _p(cb), C.byref(nb))
     return (rs, cs), (rb, cb)
 
 

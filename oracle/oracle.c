@@ -236,9 +236,161 @@ static float rest_len4(const float* verts4, uint32_t i, uint32_t j) {
   return vlen(d);
 }
 
-int or_springs_from_mesh(uint32_t nv, const float* verts4, uint32_t nf, const uint32_t* faces,
-                         uint32_t* row_s, or_spring* col_s, uint32_t* nnz_s,
-                         uint32_t* row_b, or_spring* col_b, uint32_t* nnz_b) {
+/* kd_create / kd_insert3f / kd_nearest3f (kdtree.cpp:91-216, 395-520,
+   820-850), recursive as there: nodes in insertion order, a point goes left
+   when strictly below the node's coordinate on axis depth mod 3, double
+   coordinates, the bounding box grown by every insertion; a search starts
+   from the root as best, recurses into the nearer side (left when q <= the
+   node's coordinate), tests the node (strictly closer), then the farther
+   side if its box is strictly closer than the best. */
+typedef struct { double p[3]; uint32_t data; int dir; int32_t l, r; } kdn;
+typedef struct { kdn* n; uint32_t cnt, cap; double mn[3], mx[3]; } kdt;
+static void kd_ins(kdt* t, const float* pf, uint32_t data) {
+  if (t->cnt == t->cap) { t->cap = t->cap ? 2 * t->cap : 64; t->n = (kdn*)realloc(t->n, sizeof(kdn) * t->cap); }
+  kdn* x = &t->n[t->cnt];
+  for (int c = 0; c < 3; ++c) x->p[c] = (double)pf[c];
+  x->data = data; x->l = x->r = -1; x->dir = 0;
+  const int32_t id = (int32_t)t->cnt++;
+  if (id == 0) { for (int c = 0; c < 3; ++c) t->mn[c] = t->mx[c] = x->p[c]; return; }
+  int32_t cur = 0;
+  for (;;) {
+    kdn* c = &t->n[cur];
+    int32_t* link = (t->n[id].p[c->dir] < c->p[c->dir]) ? &c->l : &c->r;
+    if (*link < 0) { *link = id; t->n[id].dir = (c->dir + 1) % 3; break; }
+    cur = *link;
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (t->n[id].p[c] < t->mn[c]) t->mn[c] = t->n[id].p[c];
+    if (t->n[id].p[c] > t->mx[c]) t->mx[c] = t->n[id].p[c];
+  }
+}
+static double kd_d2(const kdn* x, const double* q) {
+  double d = 0; for (int c = 0; c < 3; ++c) d += (x->p[c] - q[c]) * (x->p[c] - q[c]); return d;
+}
+static void kd_near_rec(const kdt* t, int32_t i, const double* q, int32_t* best, double* bd, double* mn, double* mx) {
+  const kdn* x = &t->n[i];
+  const int dir = x->dir;
+  int32_t nearer, farther; double *nc, *fc;
+  if (q[dir] - x->p[dir] <= 0) { nearer = x->l; farther = x->r; nc = mx + dir; fc = mn + dir; }
+  else { nearer = x->r; farther = x->l; nc = mn + dir; fc = mx + dir; }
+  if (nearer >= 0) { double sv = *nc; *nc = x->p[dir]; kd_near_rec(t, nearer, q, best, bd, mn, mx); *nc = sv; }
+  double d = kd_d2(x, q);
+  if (d < *bd) { *best = i; *bd = d; }
+  if (farther >= 0) {
+    double sv = *fc; *fc = x->p[dir];
+    double r = 0;
+    for (int c = 0; c < 3; ++c) {
+      if (q[c] < mn[c]) r += (mn[c] - q[c]) * (mn[c] - q[c]);
+      else if (q[c] > mx[c]) r += (mx[c] - q[c]) * (mx[c] - q[c]);
+    }
+    if (r < *bd) kd_near_rec(t, farther, q, best, bd, mn, mx);
+    *fc = sv;
+  }
+}
+/* data of the nearest point; the tree must not be empty */
+static uint32_t kd_nearest_of(const kdt* t, const float* qf) {
+  const double q[3] = {qf[0], qf[1], qf[2]};
+  double mn[3], mx[3];
+  memcpy(mn, t->mn, sizeof mn); memcpy(mx, t->mx, sizeof mx);
+  int32_t best = 0; double bd = kd_d2(&t->n[0], q);
+  kd_near_rec(t, 0, q, &best, &bd, mn, mx);
+  return t->n[best].data;
+}
+
+int32_t or_kd_nearest(uint32_t n, const float* pts3, uint32_t nq, const float* q3, uint32_t* out) {
+  if (n == 0) return -1;
+  kdt t = {0};
+  for (uint32_t i = 0; i < n; ++i) kd_ins(&t, pts3 + 3 * (size_t)i, i);
+  for (uint32_t k = 0; k < nq; ++k) out[k] = kd_nearest_of(&t, q3 + 3 * (size_t)k);
+  free(t.n);
+  return 0;
+}
+
+/* Springs::get_cloth_boundary_spring / get_boundary_boundary_spring
+   (spring.cpp:98-209).  Returns the number of cloth-boundary pairs (cb,
+   2 per pair, may be NULL to count) and writes the boundary set's pairs,
+   sorted and unique, to bd (count in *nbd); negative for the inputs the
+   reference reads out of range: -1 odd group count, -2 groups past nv,
+   -3 an empty piece, -4 fewer than 100 boundary vertices, -5 a boundary
+   with no other boundary. */
+static int cmp_pair(const void* a, const void* b) {
+  const uint32_t* x = (const uint32_t*)a; const uint32_t* y = (const uint32_t*)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+  return 0;
+}
+int32_t or_boundary_springs(uint32_t nv, const float* verts4, uint32_t ngroups, const uint32_t* counts,
+                            uint32_t* cb, uint32_t* bd, uint32_t* nbd) {
+  if (ngroups % 2) return -1;
+  uint64_t tot = 0;
+  for (uint32_t g = 0; g < ngroups; ++g) tot += counts[g];
+  if (tot > nv) return -2;
+  float* v3 = (float*)malloc(sizeof(float) * 3 * (size_t)(nv ? nv : 1));
+  for (uint32_t i = 0; i < nv; ++i) memcpy(v3 + 3 * (size_t)i, verts4 + 4 * (size_t)i, 12);
+  uint32_t ncb = 0, start = 0;
+  uint32_t* pairs = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (size_t)(nv + 1));
+  uint32_t* rs = (uint32_t*)malloc(sizeof(uint32_t) * (ngroups + 2));
+  uint32_t* re = (uint32_t*)malloc(sizeof(uint32_t) * (ngroups + 2));
+  uint32_t nr = 0;
+  for (uint32_t g = 0; g < ngroups; g += 2) {
+    if (counts[g] == 0) { free(v3); free(pairs); free(rs); free(re); return -3; }
+    kdt t = {0};
+    for (uint32_t i = 0; i < counts[g]; ++i) kd_ins(&t, v3 + 3 * (size_t)(start + i), start + i);
+    start += counts[g];
+    for (uint32_t i = 0; i < counts[g + 1]; ++i) {
+      pairs[2 * ncb] = start + i;
+      pairs[2 * ncb + 1] = kd_nearest_of(&t, v3 + 3 * (size_t)(start + i));
+      ++ncb;
+    }
+    free(t.n);
+    rs[nr] = start; re[nr] = start + counts[g + 1]; ++nr;
+    start += counts[g + 1];
+  }
+  if (ncb < 100) { free(v3); free(pairs); free(rs); free(re); return -4; }
+  float md = 0.0f;
+  for (int i = 0; i < 100; ++i) md += rest_len4(verts4, pairs[2 * i], pairs[2 * i + 1]);
+  md /= 100u;
+  /* boundary vertex -> its piece vertex */
+  uint32_t* piece = (uint32_t*)calloc(nv ? nv : 1, sizeof(uint32_t));
+  for (uint32_t i = 0; i < ncb; ++i) piece[pairs[2 * i]] = pairs[2 * i + 1];
+  uint32_t nb = 0, capb = 64;
+  uint32_t* bb = (uint32_t*)malloc(sizeof(uint32_t) * 2 * capb);
+  for (uint32_t i = 0; i < nr; ++i) {
+    kdt t = {0};
+    for (uint32_t j = 0; j < nr; ++j) {
+      if (j == i) continue;
+      for (uint32_t k = rs[j]; k < re[j]; ++k) kd_ins(&t, v3 + 3 * (size_t)k, k);
+    }
+    if (rs[i] < re[i] && t.cnt == 0) {
+      free(t.n); free(v3); free(pairs); free(rs); free(re); free(piece); free(bb); return -5;
+    }
+    for (uint32_t k = rs[i]; k < re[i]; ++k) {
+      uint32_t r = kd_nearest_of(&t, v3 + 3 * (size_t)k);
+      float d = rest_len4(verts4, k, r);
+      if (d < md * 50 && d > 0) {
+        if (nb == capb) { capb *= 2; bb = (uint32_t*)realloc(bb, sizeof(uint32_t) * 2 * capb); }
+        bb[2 * nb] = piece[k]; bb[2 * nb + 1] = piece[r]; ++nb;
+      }
+    }
+    free(t.n);
+  }
+  qsort(bb, nb, 2 * sizeof(uint32_t), cmp_pair);
+  uint32_t u = 0;
+  for (uint32_t i = 0; i < nb; ++i)
+    if (u == 0 || bb[2 * i] != bb[2 * (u - 1)] || bb[2 * i + 1] != bb[2 * (u - 1) + 1]) {
+      bb[2 * u] = bb[2 * i]; bb[2 * u + 1] = bb[2 * i + 1]; ++u;
+    }
+  if (cb) memcpy(cb, pairs, sizeof(uint32_t) * 2 * ncb);
+  if (bd) memcpy(bd, bb, sizeof(uint32_t) * 2 * u);
+  *nbd = u;
+  free(v3); free(pairs); free(rs); free(re); free(piece); free(bb);
+  return (int32_t)ncb;
+}
+
+int or_springs_from_mesh_ex(uint32_t nv, const float* verts4, uint32_t nf, const uint32_t* faces,
+                            uint32_t ncb, const uint32_t* cb, uint32_t nbd, const uint32_t* bd,
+                            uint32_t* row_s, or_spring* col_s, uint32_t* nnz_s,
+                            uint32_t* row_b, or_spring* col_b, uint32_t* nnz_b) {
   ulist* n1 = (ulist*)calloc(nv, sizeof(ulist));
   ulist* n2 = (ulist*)calloc(nv, sizeof(ulist));
   for (uint32_t f = 0; f < nf; ++f) {           /* spring.cpp:290-312 */
@@ -247,6 +399,11 @@ int or_springs_from_mesh(uint32_t nv, const float* verts4, uint32_t nf, const ui
     for (int q = 0; q < 6; ++q)
       if (!ul_has(&n1[t[ord[q][0]]], t[ord[q][1]])) ul_push(&n1[t[ord[q][0]]], t[ord[q][1]]);
   }
+  for (uint32_t i = 0; i < ncb; ++i) {          /* spring.cpp:314-323 */
+    ul_push(&n1[cb[2 * i]], cb[2 * i + 1]);
+    ul_push(&n1[cb[2 * i + 1]], cb[2 * i]);
+  }
+  for (uint32_t i = 0; i < nbd; ++i) ul_push(&n1[bd[2 * i]], bd[2 * i + 1]);
   /* spring.cpp:328-351 with Matrix::Insert_Matrix (:30-79): an edge seen
      again (either direction) pairs its first opposite vertex with this one */
   umap mat; umap_init(&mat, (size_t)nf * 3);
@@ -292,6 +449,13 @@ int or_springs_from_mesh(uint32_t nv, const float* verts4, uint32_t nf, const ui
   for (uint32_t i = 0; i < nv; ++i) { free(n1[i].v); free(n2[i].v); }
   free(n1); free(n2);
   return 0;
+}
+
+int or_springs_from_mesh(uint32_t nv, const float* verts4, uint32_t nf, const uint32_t* faces,
+                         uint32_t* row_s, or_spring* col_s, uint32_t* nnz_s,
+                         uint32_t* row_b, or_spring* col_b, uint32_t* nnz_b) {
+  return or_springs_from_mesh_ex(nv, verts4, nf, faces, 0, NULL, 0, NULL, row_s, col_s, nnz_s, row_b, col_b,
+                                 nnz_b);
 }
 
 /* a5: the first face corner that visits a vertex decides its normal */

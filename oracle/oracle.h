@@ -101,6 +101,19 @@ uint32_t or_icosphere(uint32_t level, double radius, const double c[3],
 int or_springs_from_mesh(uint32_t nv, const float* verts4, uint32_t nf, const uint32_t* faces,
                          uint32_t* row_s, or_spring* col_s, uint32_t* nnz_s,
                          uint32_t* row_b, or_spring* col_b, uint32_t* nnz_b);
+/* the same with extra structural pairs appended as Springs::create_neigh
+   does (spring.cpp:314-323): cloth-boundary pairs both ways, then the
+   boundary set's pairs one way */
+int or_springs_from_mesh_ex(uint32_t nv, const float* verts4, uint32_t nf, const uint32_t* faces,
+                            uint32_t ncb, const uint32_t* cb, uint32_t nbd, const uint32_t* bd,
+                            uint32_t* row_s, or_spring* col_s, uint32_t* nnz_s,
+                            uint32_t* row_b, or_spring* col_b, uint32_t* nnz_b);
+/* nearest of each query among pts in the reference kd-tree's order (ties
+   as its search meets them); -1 for no points */
+int32_t or_kd_nearest(uint32_t n, const float* pts3, uint32_t nq, const float* q3, uint32_t* out);
+/* boundary springs of the vertex groups (spring.cpp:98-209): see oracle.c */
+int32_t or_boundary_springs(uint32_t nv, const float* verts4, uint32_t ngroups, const uint32_t* counts,
+                            uint32_t* cb, uint32_t* bd, uint32_t* nbd);
 
 /* --- a5: Mesh::vertex_extend (Mesh.cpp:101-117) --- */
 void or_vertex_extend(uint32_t nv, float* verts4, uint32_t nf, const uint32_t* faces,

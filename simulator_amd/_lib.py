@@ -64,7 +64,8 @@ class cc_obj(C.Structure):
     _fields_ = [("nv", C.c_uint32), ("nvn", C.c_uint32), ("nvt", C.c_uint32), ("nf", C.c_uint32),
                 ("verts4", C.POINTER(C.c_float)), ("normals3", C.POINTER(C.c_float)),
                 ("tex2", C.POINTER(C.c_float)), ("vidx", C.POINTER(C.c_uint32)),
-                ("tidx", C.POINTER(C.c_uint32)), ("nidx", C.POINTER(C.c_uint32))]
+                ("tidx", C.POINTER(C.c_uint32)), ("nidx", C.POINTER(C.c_uint32)),
+                ("ngroups", C.c_uint32), ("group_counts", C.POINTER(C.c_uint32))]
 
 
 class cc_body_desc(C.Structure):
@@ -92,6 +93,7 @@ PROTOTYPES = {
     "cc_last_error": (C.c_char_p, []),
     "cc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "cc_springs_from_mesh": (C.c_int, [P, u32, P, u32, C.POINTER(cc_csr), C.POINTER(cc_csr)]),
+    "cc_springs_from_mesh_groups": (C.c_int, [P, u32, P, u32, P, u32, C.POINTER(cc_csr), C.POINTER(cc_csr)]),
     "cc_springs_grid": (C.c_int, [u32, u32, P, C.POINTER(cc_csr)]),
     "cc_csr_free": (None, [C.POINTER(cc_csr)]),
     "cc_scene_grid_cloth": (C.c_int, [u32, u32, f64, f64, f64, u64, P]),

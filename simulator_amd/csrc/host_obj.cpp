@@ -3,6 +3,10 @@
 // set-up steps its scenes apply (Mesh::unified, get_center, scale,
 // translate, rotation; Mesh.cpp:17-38, 40-92, 119-131).
 //
+// Vertex groups ("# object name" / "# 1234 vertices" lines, as some exporters
+// write them; Meshlab's "# Object" and "# 1234 vertices," do not match, as in
+// the reference) are kept for the boundary springs.
+//
 // The reader keeps the reference's reading of each line, including what it
 // does with inputs it was not written for: a face keeps its first three
 // corners; "f a b c" faces take the normal index from the vertex index and
@@ -62,7 +66,7 @@ int cc_obj_read(const char* path, cc_obj* out) {
     std::ifstream input(path);
     CC_REQUIRE((bool)input, std::string("cc_obj_read: unable to open ") + path);
     std::vector<float> v4, n3, t2;
-    std::vector<uint32_t> vi, ti, ni;
+    std::vector<uint32_t> vi, ti, ni, groups;
     std::string buffer;
     uint64_t lineno = 0;
     while (std::getline(input, buffer)) {
@@ -70,7 +74,14 @@ int cc_obj_read(const char* path, cc_obj* out) {
       std::stringstream line(buffer);
       std::string f0, f1, f2, f3;
       line >> f0 >> f1 >> f2 >> f3;
-      if (f0 == "v") {                       // glm::vec4(atof, atof, atof, 1.0)
+      if (f0 == "mtllib") {                  // material file: rendering only
+      } else if (f1 == "object") {           // a vertex group opens ("# object name")
+        groups.push_back(0);
+      } else if (f2 == "vertices") {         // its vertex count ("# 1234 vertices")
+        CC_REQUIRE(!groups.empty(), "cc_obj_read: vertex count before any object line at line " +
+                                        std::to_string(lineno));
+        groups.back() = (uint32_t)std::atoi(f1.c_str());
+      } else if (f0 == "v") {                // glm::vec4(atof, atof, atof, 1.0)
         v4.insert(v4.end(), {(float)std::atof(f1.c_str()), (float)std::atof(f2.c_str()),
                              (float)std::atof(f3.c_str()), 1.0f});
       } else if (f0 == "vn") {
@@ -109,6 +120,8 @@ int cc_obj_read(const char* path, cc_obj* out) {
     out->vidx = copy_out(vi);
     out->tidx = copy_out(ti);
     out->nidx = copy_out(ni);
+    out->ngroups = (uint32_t)groups.size();
+    out->group_counts = copy_out(groups);
   });
 }
 
@@ -120,6 +133,7 @@ void cc_obj_free(cc_obj* obj) {
   std::free(obj->vidx);
   std::free(obj->tidx);
   std::free(obj->nidx);
+  std::free(obj->group_counts);
   std::memset(obj, 0, sizeof(*obj));
 }
 

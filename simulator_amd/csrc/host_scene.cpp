@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <set>
 #include <unordered_map>
 
 #include "cc_internal.h"
@@ -45,6 +47,188 @@ static void fill_csr(const std::vector<std::vector<uint32_t>>& neigh, const floa
     }
   }
   out->row[n] = k;
+}
+
+// The nearest-neighbour search of the reference's boundary springs
+// (kd_create / kd_insert3f / kd_nearest3f of its kdtree.cpp), restated for
+// its results, ties included: an unbalanced 3-d tree in insertion order
+// (split axis = depth mod 3, a point goes left when strictly below the
+// node's coordinate), double coordinates, a bounding box grown by every
+// insertion.  A search takes the root as its first best, then at each node
+// visits the nearer side (left when q <= the node's coordinate), tests the
+// node itself (strictly closer wins), then the farther side if the box of
+// that side is strictly closer than the best.  So among equally near points
+// the first one reached in that order wins, and the root wins any tie with
+// it.  The walk uses an explicit stack: insertion-ordered meshes give deep
+// trees.
+class KdTree {
+ public:
+  void insert(const float* p, uint32_t data) {
+    Node nd;
+    for (int c = 0; c < 3; ++c) nd.p[c] = (double)p[c];
+    nd.data = data;
+    if (nodes_.empty()) {
+      nd.dir = 0;
+      nodes_.push_back(nd);
+      for (int c = 0; c < 3; ++c) rmin_[c] = rmax_[c] = nd.p[c];
+      return;
+    }
+    int32_t cur = 0;
+    for (;;) {
+      const Node& c = nodes_[cur];
+      const bool left = nd.p[c.dir] < c.p[c.dir];
+      const int32_t next = left ? c.left : c.right;
+      if (next >= 0) {
+        cur = next;
+        continue;
+      }
+      nd.dir = (c.dir + 1) % 3;
+      const int32_t id = (int32_t)nodes_.size();
+      nodes_.push_back(nd);
+      (left ? nodes_[cur].left : nodes_[cur].right) = id;
+      break;
+    }
+    for (int c = 0; c < 3; ++c) {
+      if (nd.p[c] < rmin_[c]) rmin_[c] = nd.p[c];
+      if (nd.p[c] > rmax_[c]) rmax_[c] = nd.p[c];
+    }
+  }
+
+  bool empty() const { return nodes_.empty(); }
+
+  uint32_t nearest(const float* qf) const {
+    const double q[3] = {(double)qf[0], (double)qf[1], (double)qf[2]};
+    int32_t best = 0;
+    double best_d = dist_sq(nodes_[0], q);
+    double mn[3] = {rmin_[0], rmin_[1], rmin_[2]}, mx[3] = {rmax_[0], rmax_[1], rmax_[2]};
+    struct Frame {
+      int32_t n;
+      int stage;       // 0: enter, 1: nearer side done, 2: farther side done
+      double saved;    // the box coordinate the side's visit replaced
+    };
+    std::vector<Frame> st;
+    st.push_back({0, 0, 0.0});
+    while (!st.empty()) {
+      Frame f = st.back();
+      st.pop_back();
+      const Node& nd = nodes_[f.n];
+      const int dir = nd.dir;
+      const bool toward_left = q[dir] - nd.p[dir] <= 0;
+      const int32_t nearer = toward_left ? nd.left : nd.right;
+      const int32_t farther = toward_left ? nd.right : nd.left;
+      double* near_c = toward_left ? &mx[dir] : &mn[dir];
+      double* far_c = toward_left ? &mn[dir] : &mx[dir];
+      if (f.stage == 0) {
+        if (nearer >= 0) {
+          st.push_back({f.n, 1, *near_c});
+          *near_c = nd.p[dir];
+          st.push_back({nearer, 0, 0.0});
+          continue;
+        }
+        f.stage = 1;
+        f.saved = *near_c;        // nothing was replaced: restoring is a no-op
+      }
+      if (f.stage == 1) {
+        *near_c = f.saved;
+        const double d = dist_sq(nd, q);
+        if (d < best_d) {
+          best = f.n;
+          best_d = d;
+        }
+        if (farther >= 0) {
+          const double sv = *far_c;
+          *far_c = nd.p[dir];
+          if (box_dist_sq(mn, mx, q) < best_d) {
+            st.push_back({f.n, 2, sv});
+            st.push_back({farther, 0, 0.0});
+            continue;
+          }
+          *far_c = sv;
+        }
+        continue;
+      }
+      *far_c = f.saved;            // stage 2
+    }
+    return nodes_[best].data;
+  }
+
+ private:
+  struct Node {
+    double p[3];
+    uint32_t data;
+    int dir;
+    int32_t left = -1, right = -1;
+  };
+  static double dist_sq(const Node& n, const double* q) {
+    double d = 0;
+    for (int c = 0; c < 3; ++c) d += (n.p[c] - q[c]) * (n.p[c] - q[c]);
+    return d;
+  }
+  static double box_dist_sq(const double* mn, const double* mx, const double* q) {
+    double d = 0;
+    for (int c = 0; c < 3; ++c) {
+      if (q[c] < mn[c])
+        d += (mn[c] - q[c]) * (mn[c] - q[c]);
+      else if (q[c] > mx[c])
+        d += (mx[c] - q[c]) * (mx[c] - q[c]);
+    }
+    return d;
+  }
+  std::vector<Node> nodes_;
+  double rmin_[3] = {0, 0, 0}, rmax_[3] = {0, 0, 0};
+};
+
+// Springs::get_cloth_boundary_spring and get_boundary_boundary_spring
+// (spring.cpp:98-209) over the OBJ vertex groups, taken in pairs {cloth
+// piece, its boundary}: every boundary vertex is tied to its nearest vertex
+// of its piece (cloth_boundary, in group and vertex order); every boundary
+// vertex's nearest vertex on the other boundaries, when 0 < distance < 50 x
+// the mean length of the first 100 of those ties, gives a pair of the two
+// boundary vertices' piece vertices (the sorted set `boundary`).  Inputs the
+// reference reads out of range or through a null search result are
+// rejected: an odd number of groups, groups past the vertex count, an empty
+// piece, fewer than 100 boundary vertices, a single boundary.
+void boundary_springs(const float* verts4, uint32_t nv, const uint32_t* counts, uint32_t ngroups,
+                      std::vector<std::pair<uint32_t, uint32_t>>& cloth_boundary,
+                      std::set<std::pair<uint32_t, uint32_t>>& boundary) {
+  CC_REQUIRE(ngroups % 2 == 0, "boundary springs: the vertex groups must come in pairs (piece, boundary)");
+  uint64_t total = 0;
+  for (uint32_t g = 0; g < ngroups; ++g) total += counts[g];
+  CC_REQUIRE(total <= nv, "boundary springs: the vertex groups cover more vertices than the mesh has");
+  uint32_t g_start = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> ranges;     // the boundaries' vertex ranges
+  for (uint32_t g = 0; g < ngroups; g += 2) {
+    CC_REQUIRE(counts[g] > 0, "boundary springs: a cloth piece without vertices");
+    KdTree kd;
+    for (uint32_t i = 0; i < counts[g]; ++i) kd.insert(verts4 + 4 * (size_t)(g_start + i), g_start + i);
+    g_start += counts[g];
+    for (uint32_t i = 0; i < counts[g + 1]; ++i)
+      cloth_boundary.emplace_back(g_start + i, kd.nearest(verts4 + 4 * (size_t)(g_start + i)));
+    ranges.emplace_back(g_start, g_start + counts[g + 1]);
+    g_start += counts[g + 1];
+  }
+  CC_REQUIRE(cloth_boundary.size() >= 100,
+             "boundary springs: fewer than 100 boundary vertices (the reference averages the first 100)");
+  float max_dist = 0.0f;
+  for (int i = 0; i < 100; ++i)
+    max_dist += rest_length(verts4 + 4 * (size_t)cloth_boundary[i].first, verts4 + 4 * (size_t)cloth_boundary[i].second);
+  max_dist /= 100u;
+  std::map<uint32_t, uint32_t> to_piece;
+  for (auto& cb : cloth_boundary) to_piece[cb.first] = cb.second;
+  for (size_t i = 0; i < ranges.size(); ++i) {
+    KdTree kd;
+    for (size_t j = 0; j < ranges.size(); ++j) {
+      if (j == i) continue;
+      for (uint32_t k = ranges[j].first; k < ranges[j].second; ++k) kd.insert(verts4 + 4 * (size_t)k, k);
+    }
+    if (ranges[i].first == ranges[i].second) continue;
+    CC_REQUIRE(!kd.empty(), "boundary springs: a single boundary has no other boundary to meet");
+    for (uint32_t k = ranges[i].first; k < ranges[i].second; ++k) {
+      const uint32_t r = kd.nearest(verts4 + 4 * (size_t)k);
+      const float d = rest_length(verts4 + 4 * (size_t)k, verts4 + 4 * (size_t)r);
+      if (d < max_dist * 50 && d > 0) boundary.insert({to_piece[k], to_piece[r]});
+    }
+  }
 }
 
 }  // namespace cc
@@ -92,10 +276,12 @@ void cc_csr_free(cc_csr* c) {
   c->n = c->nnz = 0;
 }
 
-// Springs::create_neigh (spring.cpp:286-352) + CSR_spring (:374-391).
-int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
-                         cc_csr* out_s, cc_csr* out_b) {
-  return guard([&] {
+// Springs::Springs (spring.cpp:82-91): the boundary springs of a
+// SINGLE_LAYER_BOUNDARY mesh (when groups are given), then create_neigh
+// (:286-352) + CSR_spring (:374-391).
+static void springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                              const uint32_t* group_counts, uint32_t ngroups, cc_csr* out_s, cc_csr* out_b) {
+  {
     CC_REQUIRE(verts4 && faces && out_s && out_b, "cc_springs_from_mesh: null argument");
     for (size_t i = 0; i < 3 * (size_t)nf; ++i)
       CC_REQUIRE(faces[i] < nv, "cc_springs_from_mesh: face index out of range");
@@ -109,6 +295,16 @@ int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces
       add1(t[0], t[1]); add1(t[0], t[2]);
       add1(t[1], t[0]); add1(t[1], t[2]);
       add1(t[2], t[0]); add1(t[2], t[1]);
+    }
+    if (group_counts) {      // spring.cpp:314-323: appended without the duplicate test
+      std::vector<std::pair<uint32_t, uint32_t>> cloth_boundary;
+      std::set<std::pair<uint32_t, uint32_t>> boundary;
+      boundary_springs(verts4, nv, group_counts, ngroups, cloth_boundary, boundary);
+      for (auto& cb : cloth_boundary) {
+        n1[cb.first].push_back(cb.second);
+        n1[cb.second].push_back(cb.first);
+      }
+      for (auto& b : boundary) n1[b.first].push_back(b.second);
     }
     // Matrix::Insert_Matrix (spring.cpp:30-79): the first time an edge is
     // seen (either direction) its opposite vertex is remembered; every later
@@ -132,6 +328,19 @@ int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces
     }
     fill_csr(n1, verts4, 4, out_s);
     fill_csr(n2, verts4, 4, out_b);
+  }
+}
+
+int cc_springs_from_mesh(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                         cc_csr* out_s, cc_csr* out_b) {
+  return guard([&] { springs_from_mesh(verts4, nv, faces, nf, nullptr, 0, out_s, out_b); });
+}
+
+int cc_springs_from_mesh_groups(const float* verts4, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                                const uint32_t* group_counts, uint32_t ngroups, cc_csr* out_s, cc_csr* out_b) {
+  return guard([&] {
+    CC_REQUIRE(group_counts || ngroups == 0, "cc_springs_from_mesh_groups: null group counts");
+    springs_from_mesh(verts4, nv, faces, nf, group_counts ? group_counts : &ngroups, ngroups, out_s, out_b);
   });
 }
 

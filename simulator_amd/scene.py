@@ -67,13 +67,22 @@ def grid_springs(nx, nz, pos):
     return res
 
 
-def springs_from_mesh(verts4, faces):
-    """Springs::Springs + CSR_structure_spring / CSR_bend_spring (spring.cpp:82-391)."""
+def springs_from_mesh(verts4, faces, vertex_object=None):
+    """Springs::Springs + CSR_structure_spring / CSR_bend_spring
+    (spring.cpp:82-391): a SINGLE_LAYER_NOB mesh, or, given the vertex
+    counts of its groups (ObjLoader.vertex_object, pairs {piece, boundary}),
+    a SINGLE_LAYER_BOUNDARY mesh with its boundary springs (:98-209)."""
     verts4 = np.ascontiguousarray(verts4, np.float32)
     faces = np.ascontiguousarray(faces, np.uint32)
     s, b = L.cc_csr(), L.cc_csr()
-    L.check(L.lib().cc_springs_from_mesh(L.ptr(verts4), verts4.shape[0], L.ptr(faces),
-                                         faces.size // 3, C.byref(s), C.byref(b)))
+    if vertex_object is None:
+        L.check(L.lib().cc_springs_from_mesh(L.ptr(verts4), verts4.shape[0], L.ptr(faces),
+                                             faces.size // 3, C.byref(s), C.byref(b)))
+    else:
+        g = np.ascontiguousarray(vertex_object, np.uint32)
+        L.check(L.lib().cc_springs_from_mesh_groups(L.ptr(verts4), verts4.shape[0], L.ptr(faces),
+                                                    faces.size // 3, L.ptr(g), g.size, C.byref(s),
+                                                    C.byref(b)))
     res = (_csr_to_numpy(s), _csr_to_numpy(b))
     L.lib().cc_csr_free(C.byref(s))
     L.lib().cc_csr_free(C.byref(b))

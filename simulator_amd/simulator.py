@@ -19,7 +19,8 @@ class ObjLoader:
     """ObjLoader (src/ObjLoader.h:19-38, ObjLoader.cpp:15-216), geometry
     only, through cc_obj_read: vertices (n, 4), normals (nvn, 3), tex
     (nvt, 2) and faces as vertex / tex / normal index triples (0-based,
-    unsigned as the reference stores them)."""
+    unsigned as the reference stores them), and the vertex counts of its
+    "# object" groups (vertex_object)."""
 
     def __init__(self, file):
         o = L.cc_obj()
@@ -35,6 +36,10 @@ class ObjLoader:
             self.vertex_index = arr(o.vidx, o.nf, 3, np.uint32)
             self.tex_index = arr(o.tidx, o.nf, 3, np.uint32)
             self.normal_index = arr(o.nidx, o.nf, 3, np.uint32)
+            # vertex counts of the "# object" groups (vertex_object, without
+            # the names), in file order
+            self.vertex_object = (np.ctypeslib.as_array(o.group_counts, shape=(o.ngroups,)).astype(np.uint32)
+                                  if o.ngroups else np.zeros(0, np.uint32))
             # per-vertex normals / uv as Mesh::unified makes them (None when a
             # face names a vertex the file does not have)
             self.unified_normals = self.unified_tex = None
