@@ -27,7 +27,11 @@
  *      zero contact normal and skip collision (same effect as
  *      Simulator::update_vertex, simulation.cu:253-257, applied every step);
  *   6. m == 1: the single leaf box is tested directly (the reference indexes
- *      internal node 0 which does not exist, cudaBRT.cu:181-183, 290-293).
+ *      internal node 0 which does not exist, cudaBRT.cu:181-183, 290-293);
+ *   7. boundary springs (spring.cpp:98-209) reject what the reference reads
+ *      out of range or through a null kd-tree result: an odd group count,
+ *      groups past the vertex count, an empty piece, fewer than 100
+ *      boundary vertices, a boundary with no other boundary.
  *
  * Child / parent encoding follows the reference (cudaBRT.h:10-33):
  *   child > 0 -> internal node (v-1); child < 0 -> leaf -(v+1); 0 -> null;
