@@ -83,8 +83,9 @@ def copy_probe(nbytes):
 def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
     """The oracle's FP32 path (the reference algorithm on the host) on a
     bounded sample of the same workload: whole steps (deform + refit or
-    rebuild + step; the step alone for a static body) until ~budget_s of CPU
-    time is spent."""
+    rebuild + step; the step alone for a static body) from the initial
+    state, one warm-up step, then three runs of the same number of steps
+    (sized so the three take about budget_s), the median reported."""
     import oracle as O
     from simulator_amd import _lib as L
     O.set_threads(nthreads)            # boxes, deformation and refit subtrees; the step takes nthreads
@@ -93,52 +94,58 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
     rest = O.vertex_extend(v4, b.tris, b.tris, b.normals, sc.params.body_extend)[:, :3].copy()
     caps = b.caps
     d = O.bvh_build(rest, b.tris, sc.params.box_depth)
-    h = O.BvhHandle(d)
     prm = O.params_from(sc.params)
     pins = np.zeros(sc.n, np.uint8)
     pins[sc.pins] = 1
-    x, xp, c = sc.pos.copy(), sc.pos.copy(), np.zeros_like(sc.pos)
     m = b.motion
-    steps = 0
-    t0 = time.perf_counter()
     moving = b.mode in (L.CC_BODY_REFIT, L.CC_BODY_REBUILD)
-    while True:
-        if not moving:
-            x, xp, c = O.step_f32(x, xp, c, sc.springs, prm, pins, h, nthreads=nthreads)
-            steps += 1
-            el = time.perf_counter() - t0
-            if el >= budget_s or steps >= 200:
-                break
-            continue
-        # frame motion (same formulas as the device deformation, host doubles)
-        t = steps * float(np.float32(sc.params.dt))
-        w = np.sin(2 * np.pi * m["freq"] * t)
-        th = m["rot_amp"] * w
-        R64 = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
-        P = np.array(m["pivot"])
-        T = (P - R64 @ P + m["amp"] * w * np.array(m["dir"])).astype(np.float32)
-        R = R64.astype(np.float32).ravel()
-        breath = (1 + m["breath_amp"] * np.sin(2 * np.pi * m["breath_freq"] * t
-                                               + caps["phase"].astype(np.float64))).astype(np.float32)
-        verts = O.body_deform(rest, caps["vstart"], caps["p0"], caps["p1"], breath, R, T)
-        if b.mode == L.CC_BODY_REBUILD:
-            d = O.bvh_build(verts, b.tris, sc.params.box_depth)
-        else:
-            lb = O.prim_boxes(verts, d["leaf_tri"], sc.params.box_depth)
-            d["box_int"][:] = O.refit(d["childA"], d["childB"], d["parent_int"], lb)
-            d["box_leaf"][:] = lb
-            d["verts"] = np.ascontiguousarray(verts)
-        h = O.BvhHandle(d)
-        x, xp, c = O.step_f32(x, xp, c, sc.springs, prm, pins, h, nthreads=nthreads)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 200:
-            break
+    state = {"x": sc.pos.copy(), "xp": sc.pos.copy(), "c": np.zeros_like(sc.pos), "frame": 0, "d": d,
+             "h": O.BvhHandle(d)}
+
+    def one_step():
+        if moving:
+            # frame motion (same formulas as the device deformation, host doubles)
+            t = state["frame"] * float(np.float32(sc.params.dt))
+            w = np.sin(2 * np.pi * m["freq"] * t)
+            th = m["rot_amp"] * w
+            R64 = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+            P = np.array(m["pivot"])
+            T = (P - R64 @ P + m["amp"] * w * np.array(m["dir"])).astype(np.float32)
+            R = R64.astype(np.float32).ravel()
+            breath = (1 + m["breath_amp"] * np.sin(2 * np.pi * m["breath_freq"] * t
+                                                   + caps["phase"].astype(np.float64))).astype(np.float32)
+            verts = O.body_deform(rest, caps["vstart"], caps["p0"], caps["p1"], breath, R, T)
+            dd = state["d"]
+            if b.mode == L.CC_BODY_REBUILD:
+                dd = O.bvh_build(verts, b.tris, sc.params.box_depth)
+            else:
+                lb = O.prim_boxes(verts, dd["leaf_tri"], sc.params.box_depth)
+                dd["box_int"][:] = O.refit(dd["childA"], dd["childB"], dd["parent_int"], lb)
+                dd["box_leaf"][:] = lb
+                dd["verts"] = np.ascontiguousarray(verts)
+            state["d"] = dd
+            state["h"] = O.BvhHandle(dd)
+        state["x"], state["xp"], state["c"] = O.step_f32(state["x"], state["xp"], state["c"], sc.springs, prm,
+                                                         pins, state["h"], nthreads=nthreads)
+        state["frame"] += 1
+
+    t0 = time.perf_counter()
+    one_step()                          # warm-up
+    per = time.perf_counter() - t0
+    nsteps = int(max(1, min(200, budget_s / 3.0 / max(per, 1e-6))))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            one_step()
+        runs.append(time.perf_counter() - t0)
     O.set_threads(1)
+    el = float(np.median(runs))
     what = {L.CC_BODY_REFIT: "deform + refit + ", L.CC_BODY_REBUILD: "deform + rebuild + "}.get(b.mode, "")
-    return {"value": sc.n * steps / el, "unit": "vertex-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{steps} whole steps ({what}Verlet/spring/collision) of the same {label} workload "
-                      f"from the initial state, FP32 oracle, {el:.1f} s"}
+    return {"value": sc.n * nsteps / el, "unit": "vertex-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"median of 3 runs of {nsteps} whole steps ({what}Verlet/spring/collision) of the same "
+                      f"{label} workload after 1 warm-up step from the initial state, FP32 oracle, "
+                      f"{el:.2f} s per run"}
 
 
 def batched_leg(warmup=16, steps=160, plain_steps=32):
