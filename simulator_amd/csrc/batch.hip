@@ -11,6 +11,7 @@
 // instance first) are chosen at creation by timing each candidate on a copy
 // of the initial state.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -20,7 +21,7 @@
 
 namespace cc {
 
-template <int TILE>
+template <int TILE, int EW>
 __global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__ sin_, float* __restrict__ sout,
                                                        float* __restrict__ contact, const uint32_t* __restrict__ row,
                                                        const uint2* __restrict__ rec,
@@ -32,7 +33,7 @@ __global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__
   const InstDesc d = insts[b.x];
   const uint32_t i = b.y + threadIdx.x;
   uint32_t visits = 0;
-  step_vertex<false, 0, false>(i, sin_, sout, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
+  step_vertex<false, EW, false>(i, sin_, sout, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
                      nullptr, nullptr);
 }
 
@@ -47,6 +48,8 @@ struct Batch {
   DBuf<float> state[2], contact;
   DBuf<uint32_t> row;
   DBuf<uint2> rec;
+  DBuf<uint2> ell;                           // [ELL_W * ntotal] the records, fixed width (when ell_w)
+  int ell_w = 0;
   DBuf<int32_t> spill;
   BatchBodies bodies;
   DBuf<InstDesc> insts[2];                   // per body slot (frame g uses slot g % 2 == cur)
@@ -82,19 +85,25 @@ struct Batch {
     return t;
   }
 
-  void launch(int T, const uint4* blk, uint32_t nblk, hipStream_t s) {
+  template <int T>
+  void launch_t(const uint4* blk, uint32_t nblk, hipStream_t s) {
     const float* in = state[cur].p;
     float* out = state[cur ^ 1].p;
-    if (T == 64)
-      k_verlet_batch<64><<<nblk, 64, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
-                                             spill.p);
-    else if (T == 128)
-      k_verlet_batch<128><<<nblk, 128, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
-                                               spill.p);
+    if (ell_w)
+      k_verlet_batch<T, ELL_W><<<nblk, T, 0, s>>>(in, out, contact.p, row.p, ell.p, insts[cur].p, blk, ntotal,
+                                                  spill.p);
     else
-      k_verlet_batch<256><<<nblk, 256, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
-                                               spill.p);
+      k_verlet_batch<T, 0><<<nblk, T, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
+                                              spill.p);
     CC_HIP(hipGetLastError());
+  }
+  void launch(int T, const uint4* blk, uint32_t nblk, hipStream_t s) {
+    if (T == 64)
+      launch_t<64>(blk, nblk, s);
+    else if (T == 128)
+      launch_t<128>(blk, nblk, s);
+    else
+      launch_t<256>(blk, nblk, s);
   }
 
   // Time every (tile, order) candidate over a few steps from the initial
@@ -156,8 +165,8 @@ struct Batch {
     if (bodies.moving) {
       bodies.pre(bodies.step, k, nxt, s);
       if (bodies.step.n_sort) {
-        StepLaunch st{state[cur].p, state[cur ^ 1].p, contact.p, row.p, rec.p, insts[cur].p,
-                      blocks256.p, nblocks256, ntotal, spill.p};
+        StepLaunch st{state[cur].p, state[cur ^ 1].p, contact.p, row.p, ell_w ? ell.p : rec.p, insts[cur].p,
+                      blocks256.p, nblocks256, ntotal, spill.p, (uint32_t)ell_w};
         bodies.sort_with_step(bodies.step, nxt, st, s);
       } else {
         launch(tile, blocks.p, nblocks, s);
@@ -279,6 +288,23 @@ int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, con
       B.rec.zero(s);
     } else {
       B.rec.upload(rec_all.data(), rec_all.size(), s);
+    }
+    // fixed-width copy of the records when every vertex has at most ELL_W
+    // springs (CC_SPRINGS_ELL=0 keeps the merged rows only)
+    uint32_t max_deg = 0;
+    for (uint32_t i = 0; i < B.ntotal; ++i) max_deg = std::max(max_deg, B.h_row[i + 1] - B.h_row[i]);
+    bool ell = max_deg <= (uint32_t)ELL_W;
+    if (const char* e = std::getenv("CC_SPRINGS_ELL"))
+      if (std::atoi(e) == 0) ell = false;
+    if (ell) {
+      std::vector<uint2> er((size_t)B.ntotal * ELL_W);
+      for (uint32_t i = 0; i < B.ntotal; ++i)
+        for (uint32_t k = 0; k < (uint32_t)ELL_W; ++k) {
+          const uint32_t e = B.h_row[i] + k;
+          er[(size_t)i * ELL_W + k] = e < B.h_row[i + 1] ? rec_all[e] : make_uint2(i | PAD_TYPE, 0u);
+        }
+      B.ell.upload(er.data(), er.size(), s);
+      B.ell_w = ELL_W;
     }
     B.spill.alloc((size_t)STACK_SPILL * B.ntotal);
     // bodies: concatenated, frame 0 built for all of them

@@ -276,6 +276,7 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort(const BodyDesc* __restri
 // The sort's workgroups, then one cloth step of the batch in blocks of four
 // 256-vertex tiles (the sort of frame g + 1 and the step of frame g read and
 // write disjoint buffers).
+template <int EW>
 __global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __restrict__ desc,
                                                              const uint32_t* __restrict__ list, uint32_t nsort,
                                                              const float* __restrict__ boxes,
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __r
   const InstDesc dd = st.insts[b.x];
   const uint32_t i = b.y + lt;
   uint32_t visits = 0;
-  step_vertex<false, 0, false>(i, st.in, st.out, st.contact, st.row, st.rec, dd.a, dd.bvh, Spill{st.spill + i, st.ntotal},
+  step_vertex<false, EW, false, 6>(i, st.in, st.out, st.contact, st.row, st.rec, dd.a, dd.bvh, Spill{st.spill + i, st.ntotal},
                      visits, nullptr, nullptr, nullptr);
 }
 
@@ -814,9 +815,14 @@ void BatchBodies::sort(const BodyPass& P, int slot, hipStream_t s) {
 
 void BatchBodies::sort_with_step(const BodyPass& P, int slot, const StepLaunch& st, hipStream_t s) {
   const uint32_t nstep = (st.nblocks + SB_THREADS / 256 - 1) / (SB_THREADS / 256);
-  k_bb_sort_step<<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p, kv[0].p,
-                                                          kv[1].p, keys.p, leaf_tri[slot].p, tris.p, root[slot].p,
-                                                          st);
+  if (st.ew == ELL_W)
+    k_bb_sort_step<ELL_W><<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p,
+                                                                  kv[0].p, kv[1].p, keys.p, leaf_tri[slot].p, tris.p,
+                                                                  root[slot].p, st);
+  else
+    k_bb_sort_step<0><<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p,
+                                                              kv[0].p, kv[1].p, keys.p, leaf_tri[slot].p, tris.p,
+                                                              root[slot].p, st);
   CC_HIP(hipGetLastError());
 }
 

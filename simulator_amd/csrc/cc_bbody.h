@@ -40,6 +40,7 @@ struct StepLaunch {
   uint32_t nblocks;
   uint32_t ntotal;
   int32_t* spill;
+  uint32_t ew;              // rec holds ew records per vertex (0: merged rows at row)
 };
 
 struct BatchBodies {

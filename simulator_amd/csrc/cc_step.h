@@ -37,12 +37,14 @@ __device__ inline void spring_term(f3 p, f3 vel, const float2* g, uint2 rr, bool
 // collision part is exact order so the contact set is bit-exact for the
 // pre-projection positions.  verlet.cu:129-161, 52-88, 24-50.
 // EW > 0: `rec` holds EW records per vertex (see ELL_W), loaded with the
-// vertex's own records, and their gathers go out ELL_W at a time; EW == 0:
+// vertex's own records, and their gathers go out GB at a time; EW == 0:
 // `rec` is the merged CSR and `row` its offsets, and the records load once
 // the offsets have arrived.  Both sum the same springs in the same order.
 // WIDE: the view may have the four-wide form (single bodies; the batched
 // bodies never do).
-template <bool DEBUG, int EW, bool WIDE>
+// GB: gathers issued together on the fixed-width path (fewer registers for
+// kernels that share a launch with other work).
+template <bool DEBUG, int EW, bool WIDE, int GB = ELL_W>
 __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, float* __restrict__ sout,
                                    float* __restrict__ contact, const uint32_t* __restrict__ row,
                                    const uint2* __restrict__ rec, const StepArgs& a, const BvhView& bvh,
@@ -52,7 +54,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
   // compiler otherwise issues each load at its first use, one memory round
   // trip after another.
   constexpr bool ELL = EW > 0;
-  static_assert(EW % ELL_W == 0, "fixed-width rows are whole gather batches");
+  static_assert(EW % GB == 0, "fixed-width rows are whole gather batches");
   uint2 er[ELL ? EW : 1];
   if constexpr (ELL) {
 #pragma unroll
@@ -85,19 +87,19 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
     const float ks1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k1)));
     const float ks2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.k2)));
     if constexpr (ELL) {
-      // ELL_W neighbour gathers at a time, all issued before any use
+      // GB neighbour gathers at a time, all issued before any use
 #pragma unroll
-      for (int b0 = 0; b0 < EW; b0 += ELL_W) {
-        float2 g[ELL_W][3];
+      for (int b0 = 0; b0 < EW; b0 += GB) {
+        float2 g[GB][3];
 #pragma unroll
-        for (int k = 0; k < ELL_W; ++k) {
+        for (int k = 0; k < GB; ++k) {
           const float2* sj = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)(er[b0 + k].x & END_MASK));
           g[k][0] = sj[0];
           g[k][1] = sj[1];
           g[k][2] = sj[2];
         }
 #pragma unroll
-        for (int k = 0; k < ELL_W; ++k)
+        for (int k = 0; k < GB; ++k)
           spring_term(p, vel, g[k], er[b0 + k], (er[b0 + k].x & PAD_TYPE) != PAD_TYPE, a, ks0, ks1, ks2, S);
       }
     } else {
