@@ -125,6 +125,37 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_force(
   if (threadIdx.x == 0) acc[NPARAM * blockIdx.x + 3] += (double)g[0];
 }
 
+// One spring's adjoint terms (the forward's f_ij in F_i and f_ji in F_j)
+// added to gp, gq and the parameter gradients g.
+__device__ inline void adj_spring_term(f3 pi, f3 vi, f3 lFi, f3 pj, f3 vj, f3 lFj, uint2 r, const AdjArgs& a, f3& gp,
+                                       f3& gq, float (&g)[4]) {
+  const uint32_t t = r.x >> 30;
+  const float rest0 = __uint_as_float(r.y), rest = rest0 * a.rest_scale;
+  const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
+  f3 gd, gdv;
+  float dl, L;
+  // f_ij, in F_i
+  spring_vjp(mk((pi.x - pj.x) + a.eps, (pi.y - pj.y) + a.eps, (pi.z - pj.z) + a.eps), vi - vj, ks, rest, a.kd, lFi,
+             gd, gdv, dl, L);
+  gp = mk(gp.x + fmaf(gdv.x, a.inv_dt, gd.x), gp.y + fmaf(gdv.y, a.inv_dt, gd.y), gp.z + fmaf(gdv.z, a.inv_dt, gd.z));
+  gq = mk(fmaf(-gdv.x, a.inv_dt, gq.x), fmaf(-gdv.y, a.inv_dt, gq.y), fmaf(-gdv.z, a.inv_dt, gq.z));
+  const float gk = -(L - rest) * dl;
+  g[0] += t == 0 ? gk : 0.0f;
+  g[1] += t == 1 ? gk : 0.0f;
+  g[2] += t == 2 ? gk : 0.0f;
+  g[3] += ks * rest0 * dl;
+  // f_ji, in F_j (the same type and rest length)
+  spring_vjp(mk((pj.x - pi.x) + a.eps, (pj.y - pi.y) + a.eps, (pj.z - pi.z) + a.eps), vj - vi, ks, rest, a.kd, lFj,
+             gd, gdv, dl, L);
+  gp = mk(gp.x - fmaf(gdv.x, a.inv_dt, gd.x), gp.y - fmaf(gdv.y, a.inv_dt, gd.y), gp.z - fmaf(gdv.z, a.inv_dt, gd.z));
+  gq = mk(fmaf(gdv.x, a.inv_dt, gq.x), fmaf(gdv.y, a.inv_dt, gq.y), fmaf(gdv.z, a.inv_dt, gq.z));
+}
+
+// EW > 0: `rec` holds EW records per vertex (the forward's fixed-width rows,
+// empty slots of type 3 skipped), loaded with the vertex's own data and
+// gathered six at a time; EW == 0: the merged CSR at `row`, one spring after
+// another.  The same springs in the same order either way.
+template <int EW>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_springs(
     uint32_t n, const uint32_t* __restrict__ row, const uint2* __restrict__ rec, const float* __restrict__ state,
     const float* __restrict__ lf, const float* __restrict__ locp, const float* __restrict__ locq, AdjArgs a,
@@ -132,40 +163,48 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_springs(
   const uint32_t i = blockIdx.x * ADJ_THREADS + threadIdx.x;
   float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};   // k0, k1, k2, rest_scale
   if (i < n) {
-    const uint32_t r0 = row[i] & ROW_MASK, r1 = row[i + 1] & ROW_MASK;
+    uint2 er[EW ? EW : 1];
+    if constexpr (EW > 0) {
+#pragma unroll
+      for (int k = 0; k < EW; ++k) er[k] = rec[(size_t)i * EW + k];
+    }
     const float* s = state + 6 * (size_t)i;
     const f3 pi = mk(s[0], s[1], s[2]);
     const f3 vi = mk((s[0] - s[3]) * a.inv_dt, (s[1] - s[4]) * a.inv_dt, (s[2] - s[5]) * a.inv_dt);
     const f3 lFi = ld3(lf + 3 * (size_t)i);
     f3 gp = ld3(locp + 3 * (size_t)i), gq = ld3(locq + 3 * (size_t)i);
-    for (uint32_t k = r0; k < r1; ++k) {
-      const uint2 r = rec[k];
-      const uint32_t j = r.x & END_MASK, t = r.x >> 30;
-      const float rest0 = __uint_as_float(r.y), rest = rest0 * a.rest_scale;
-      const float ks = t == 0 ? a.k0 : (t == 1 ? a.k1 : a.k2);
-      const float* sj = state + 6 * (size_t)j;
-      const f3 pj = mk(sj[0], sj[1], sj[2]);
-      const f3 vj = mk((sj[0] - sj[3]) * a.inv_dt, (sj[1] - sj[4]) * a.inv_dt, (sj[2] - sj[5]) * a.inv_dt);
-      const f3 lFj = ld3(lf + 3 * (size_t)j);
-      f3 gd, gdv;
-      float dl, L;
-      // f_ij, in F_i
-      spring_vjp(mk((pi.x - pj.x) + a.eps, (pi.y - pj.y) + a.eps, (pi.z - pj.z) + a.eps), vi - vj, ks, rest, a.kd,
-                 lFi, gd, gdv, dl, L);
-      gp = mk(gp.x + fmaf(gdv.x, a.inv_dt, gd.x), gp.y + fmaf(gdv.y, a.inv_dt, gd.y),
-              gp.z + fmaf(gdv.z, a.inv_dt, gd.z));
-      gq = mk(fmaf(-gdv.x, a.inv_dt, gq.x), fmaf(-gdv.y, a.inv_dt, gq.y), fmaf(-gdv.z, a.inv_dt, gq.z));
-      const float gk = -(L - rest) * dl;
-      g[0] += t == 0 ? gk : 0.0f;
-      g[1] += t == 1 ? gk : 0.0f;
-      g[2] += t == 2 ? gk : 0.0f;
-      g[3] += ks * rest0 * dl;
-      // f_ji, in F_j (the same type and rest length)
-      spring_vjp(mk((pj.x - pi.x) + a.eps, (pj.y - pi.y) + a.eps, (pj.z - pi.z) + a.eps), vj - vi, ks, rest, a.kd,
-                 lFj, gd, gdv, dl, L);
-      gp = mk(gp.x - fmaf(gdv.x, a.inv_dt, gd.x), gp.y - fmaf(gdv.y, a.inv_dt, gd.y),
-              gp.z - fmaf(gdv.z, a.inv_dt, gd.z));
-      gq = mk(fmaf(gdv.x, a.inv_dt, gq.x), fmaf(gdv.y, a.inv_dt, gq.y), fmaf(gdv.z, a.inv_dt, gq.z));
+    if constexpr (EW > 0) {
+      constexpr int GB = 6;
+#pragma unroll
+      for (int b0 = 0; b0 < EW; b0 += GB) {
+        float sj[GB][6], lj[GB][3];
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+          const uint32_t j = er[b0 + k].x & END_MASK;
+          const float2* s2 = reinterpret_cast<const float2*>(state + 6 * (size_t)j);
+          const float2 u = s2[0], v = s2[1], w = s2[2];
+          sj[k][0] = u.x; sj[k][1] = u.y; sj[k][2] = v.x; sj[k][3] = v.y; sj[k][4] = w.x; sj[k][5] = w.y;
+          lj[k][0] = lf[3 * (size_t)j]; lj[k][1] = lf[3 * (size_t)j + 1]; lj[k][2] = lf[3 * (size_t)j + 2];
+        }
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+          if ((er[b0 + k].x & PAD_TYPE) == PAD_TYPE) continue;
+          const f3 pj = mk(sj[k][0], sj[k][1], sj[k][2]);
+          const f3 vj = mk((sj[k][0] - sj[k][3]) * a.inv_dt, (sj[k][1] - sj[k][4]) * a.inv_dt,
+                           (sj[k][2] - sj[k][5]) * a.inv_dt);
+          adj_spring_term(pi, vi, lFi, pj, vj, mk(lj[k][0], lj[k][1], lj[k][2]), er[b0 + k], a, gp, gq, g);
+        }
+      }
+    } else {
+      const uint32_t r0 = row[i] & ROW_MASK, r1 = row[i + 1] & ROW_MASK;
+      for (uint32_t k = r0; k < r1; ++k) {
+        const uint2 r = rec[k];
+        const uint32_t j = r.x & END_MASK;
+        const float* sj = state + 6 * (size_t)j;
+        const f3 pj = mk(sj[0], sj[1], sj[2]);
+        const f3 vj = mk((sj[0] - sj[3]) * a.inv_dt, (sj[1] - sj[4]) * a.inv_dt, (sj[2] - sj[5]) * a.inv_dt);
+        adj_spring_term(pi, vi, lFi, pj, vj, ld3(lf + 3 * (size_t)j), r, a, gp, gq, g);
+      }
     }
     st3(lp_out + 3 * (size_t)i, gp);
     st3(lq_out + 3 * (size_t)i, gq);
@@ -363,8 +402,13 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
                                                    A.seg_contact.p + (j + 1) * n3, A.lp[cur].p, A.lq[cur].p, A.aa,
                                                    A.lf.p, A.locp.p, A.locq.p, A.acc.p);
         CC_HIP(hipGetLastError());
-        k_adj_springs<<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, S.rec.p, stj, A.lf.p, A.locp.p, A.locq.p, A.aa,
-                                                     A.lp[cur ^ 1].p, A.lq[cur ^ 1].p, A.acc.p);
+        if (S.ell_w == ELL_W)
+          k_adj_springs<ELL_W><<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, S.ell.p, stj, A.lf.p, A.locp.p,
+                                                              A.locq.p, A.aa, A.lp[cur ^ 1].p, A.lq[cur ^ 1].p,
+                                                              A.acc.p);
+        else
+          k_adj_springs<0><<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, S.rec.p, stj, A.lf.p, A.locp.p, A.locq.p,
+                                                          A.aa, A.lp[cur ^ 1].p, A.lq[cur ^ 1].p, A.acc.p);
         CC_HIP(hipGetLastError());
         cur ^= 1;
       }
