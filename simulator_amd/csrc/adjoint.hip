@@ -359,7 +359,7 @@ int cc_adj_forward(cc_adj* adj, const float* x0, const float* v0, const double t
         CC_HIP(hipMemcpyAsync(A.ck_state.p + c * n6, A.st[cur].p, n6 * sizeof(float), hipMemcpyDeviceToDevice, s));
         CC_HIP(hipMemcpyAsync(A.ck_contact.p + c * n3, A.contact.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
       }
-      S.launch_step(A.st[cur].p, A.st[cur ^ 1].p, A.contact.p, A.sa, 0, s);
+      S.launch_step(A.st[cur].p, A.st[cur ^ 1].p, A.contact.p, A.contact.p, A.sa, 0, s);
       cur ^= 1;
     }
     if (x_final) {
@@ -391,11 +391,9 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
       CC_HIP(hipMemcpyAsync(A.seg_state.p, A.ck_state.p + c * n6, n6 * sizeof(float), hipMemcpyDeviceToDevice, s));
       CC_HIP(hipMemcpyAsync(A.seg_contact.p, A.ck_contact.p + c * n3, n3 * sizeof(float), hipMemcpyDeviceToDevice,
                             s));
-      for (uint32_t j = 0; j < L; ++j) {
-        float* cj1 = A.seg_contact.p + (j + 1) * n3;   // the step updates the contact normals in place
-        CC_HIP(hipMemcpyAsync(cj1, A.seg_contact.p + j * n3, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
-        S.launch_step(A.seg_state.p + j * n6, A.seg_state.p + (j + 1) * n6, cj1, A.sa, 0, s);
-      }
+      for (uint32_t j = 0; j < L; ++j)   // contact normals of step j in, of step j + 1 out
+        S.launch_step(A.seg_state.p + j * n6, A.seg_state.p + (j + 1) * n6, A.seg_contact.p + j * n3,
+                      A.seg_contact.p + (j + 1) * n3, A.sa, 0, s);
       for (uint32_t j = L; j-- > 0;) {
         const float* stj = A.seg_state.p + j * n6;
         k_adj_force<<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, stj, A.seg_contact.p + j * n3,

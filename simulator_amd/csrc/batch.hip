@@ -33,7 +33,7 @@ __global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__
   const InstDesc d = insts[b.x];
   const uint32_t i = b.y + threadIdx.x;
   uint32_t visits = 0;
-  step_vertex<false, EW, false>(i, sin_, sout, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
+  step_vertex<false, EW, false>(i, sin_, sout, contact, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
                      nullptr, nullptr);
 }
 

@@ -104,8 +104,10 @@ struct Sim {
   // the fused step reading body slot k and state[cur], writing state[cur ^ 1]
   void launch_verlet(int k, hipStream_t s);
   // the plain fused step between caller buffers (no debug outputs, no
-  // counters), with the given arguments
-  void launch_step(const float* in, float* out, float* contact, const StepArgs& a, int k, hipStream_t s);
+  // counters), with the given arguments; contact normals from c_in to c_out
+  // (the same array for an update in place)
+  void launch_step(const float* in, float* out, const float* c_in, float* c_out, const StepArgs& a, int k,
+                   hipStream_t s);
 
   // Direct (not captured) update of the body pose of frame g into its slot.
   void update_direct(uint64_t g, hipStream_t s) {

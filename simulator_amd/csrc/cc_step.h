@@ -32,7 +32,8 @@ __device__ inline void spring_term(f3 p, f3 vel, const float2* g, uint2 rr, bool
 }
 
 // Vertex i from state `sin_` to `sout` (24 B {x_cur, x_prev} per vertex),
-// updating its contact normal in place.  Spring forces use FMA and rsqrt
+// its contact normal from contact_in to contact_out (the same array for an
+// update in place; a thread reads and writes only its own vertex's).  Spring forces use FMA and rsqrt
 // (positions are checked against the FP64 oracle by tolerance); the
 // collision part is exact order so the contact set is bit-exact for the
 // pre-projection positions.  verlet.cu:129-161, 52-88, 24-50.
@@ -46,7 +47,7 @@ __device__ inline void spring_term(f3 p, f3 vel, const float2* g, uint2 rr, bool
 // kernels that share a launch with other work).
 template <bool DEBUG, int EW, bool WIDE, int GB = ELL_W>
 __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, float* __restrict__ sout,
-                                   float* __restrict__ contact, const uint32_t* __restrict__ row,
+                                   const float* contact_in, float* contact_out, const uint32_t* __restrict__ row,
                                    const uint2* __restrict__ rec, const StepArgs& a, const BvhView& bvh,
                                    Spill spill, uint32_t& visits, float* __restrict__ dbg_pre,
                                    int32_t* __restrict__ dbg_hit, uint8_t* __restrict__ dbg_in) {
@@ -64,7 +65,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
   const uint32_t r0raw = row[i], r1raw = ELL ? 0u : row[i + 1];
   const float2* s2 = reinterpret_cast<const float2*>(sin_ + 6 * (size_t)i);
   const float2 e0 = s2[0], e1 = s2[1], e2 = s2[2];
-  const f3 c = ld3(contact + 3 * (size_t)i);
+  const f3 c = ld3(contact_in + 3 * (size_t)i);
   asm volatile("" ::"v"(r0raw), "v"(r1raw), "v"(e0.x), "v"(e0.y), "v"(e1.x), "v"(e1.y), "v"(e2.x), "v"(e2.y),
                "v"(c.x), "v"(c.y), "v"(c.z));
   const uint32_t r0 = r0raw & ROW_MASK, r1 = r1raw & ROW_MASK;
@@ -149,7 +150,7 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
   float* o = sout + 6 * (size_t)i;
   o[0] = np.x; o[1] = np.y; o[2] = np.z;
   o[3] = nq.x; o[4] = nq.y; o[5] = nq.z;
-  st3(contact + 3 * (size_t)i, nc);
+  st3(contact_out + 3 * (size_t)i, nc);
 }
 
 }  // namespace cc

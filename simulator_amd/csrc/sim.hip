@@ -24,7 +24,7 @@ namespace cc {
 // order so the contact set is bit-exact for the pre-projection positions.
 template <bool DEBUG, bool COUNT, int EW>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
-                                                float* __restrict__ contact, const uint32_t* __restrict__ row,
+                                                const float* contact_in, float* contact_out, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
                                                 uint32_t n, int32_t* __restrict__ spill, float* __restrict__ dbg_pre,
                                                 int32_t* __restrict__ dbg_hit, uint8_t* __restrict__ dbg_in,
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    step_vertex<DEBUG, EW, true>(i, sin_, sout, contact, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
+    step_vertex<DEBUG, EW, true>(i, sin_, sout, contact_in, contact_out, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
                             dbg_hit, dbg_in);
   }
   if (COUNT) {
@@ -126,17 +126,17 @@ StepArgs step_args(const cc_params& prm, uint32_t ntypes) {
 
 // ---------------------------------------------------------- Sim launches
 template <bool DEBUG, bool COUNT>
-static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, float* contact_io,
+static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, const float* contact_in, float* contact_out,
                             const StepArgs& a, const BvhView& v, float* dbg_pre, int32_t* dbg_hit, uint8_t* dbg_in,
                             unsigned long long* ctr) {
   if (S.ell_w == ELL_W)
-    k_verlet<DEBUG, COUNT, ELL_W><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
+    k_verlet<DEBUG, COUNT, ELL_W><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
                                                     dbg_pre, dbg_hit, dbg_in, ctr);
   else if (S.ell_w == ELL_W2)
-    k_verlet<DEBUG, COUNT, ELL_W2><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
+    k_verlet<DEBUG, COUNT, ELL_W2><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
                                                      dbg_pre, dbg_hit, dbg_in, ctr);
   else
-    k_verlet<DEBUG, COUNT, 0><<<g, 256, 0, s>>>(in, out, contact_io, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
+    k_verlet<DEBUG, COUNT, 0><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
                                                 dbg_pre, dbg_hit, dbg_in, ctr);
   CC_HIP(hipGetLastError());
 }
@@ -148,17 +148,18 @@ void Sim::launch_verlet(int k, hipStream_t s) {
   const unsigned g = grid_for(n, 256);
   const StepArgs a = args();
   if (debug && counting)
-    launch_k_verlet<true, true>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<true, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else if (debug)
-    launch_k_verlet<true, false>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<true, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else if (counting)
-    launch_k_verlet<false, true>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<false, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
   else
-    launch_k_verlet<false, false>(*this, g, s, in, out, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<false, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
 }
 
-void Sim::launch_step(const float* in, float* out, float* contact_io, const StepArgs& a, int k, hipStream_t s) {
-  launch_k_verlet<false, false>(*this, grid_for(n, 256), s, in, out, contact_io, a, body.view(k), nullptr, nullptr,
+void Sim::launch_step(const float* in, float* out, const float* c_in, float* c_out, const StepArgs& a, int k,
+                      hipStream_t s) {
+  launch_k_verlet<false, false>(*this, grid_for(n, 256), s, in, out, c_in, c_out, a, body.view(k), nullptr, nullptr,
                                 nullptr, nullptr);
 }
 
