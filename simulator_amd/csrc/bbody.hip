@@ -31,6 +31,9 @@ constexpr int SB_WAVES = SB_THREADS / 64;
 constexpr int SB_ROWS = 8;          // rows of 64 elements per wave per sort tile
 constexpr int SB_TILE = SB_THREADS * SB_ROWS;
 constexpr int SB_BATCH = 4;         // elements per thread per round of the code and key loops
+constexpr int SB_BITS = 10;         // digit width: three passes cover the 30-bit Morton codes
+constexpr int SB_DIGITS = 1 << SB_BITS;
+constexpr int SB_PASSES = 3;
 constexpr uint32_t BB_MAX_M = 1u << 19;   // triangles per batched body
 
 // Barrier ordering LDS accesses only: the global stores issued before it
@@ -103,10 +106,11 @@ __global__ __launch_bounds__(BB_BLOCK) void k_bb_boxes(const BodyDesc* __restric
 }
 
 // ------------------------------------------------------------ sort
-// One workgroup per body.  The digit counts of all four passes come from one
-// sweep over the codes.  A pass walks tiles of SB_TILE elements; wave w owns
-// rows w*SB_ROWS .. w*SB_ROWS+7 of 64 consecutive elements, ranks each row's
-// lanes by digit with 8 ballots, and keeps a running per-digit count of its
+// One workgroup per body.  Three passes of 10-bit digits (the Morton codes
+// have 30 bits); the digit counts of all passes come from one sweep over the
+// codes.  A pass walks tiles of SB_TILE elements; wave w owns rows
+// w*SB_ROWS .. w*SB_ROWS+7 of 64 consecutive elements, ranks each row's
+// lanes by digit with 10 ballots, and keeps a running per-digit count of its
 // rows in LDS, so a tile needs three barriers whatever its size (LDS-only
 // ones: a pass's scattered stores are waited for once, at its end).
 // Element order = (wave, row, lane) order, so the pass is stable.
@@ -114,9 +118,9 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
                                const float* __restrict__ partial, uint2* kv0, uint2* kv1,
                                uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri,
                                const uint32_t* __restrict__ tris, float* __restrict__ root) {
-  __shared__ uint32_t hist[4][256];
-  __shared__ uint32_t wcnt[SB_WAVES][256];
-  __shared__ uint32_t run[256];
+  __shared__ uint32_t hist[SB_PASSES][SB_DIGITS];
+  __shared__ uint32_t wcnt[SB_WAVES][SB_DIGITS];
+  __shared__ uint32_t run[SB_DIGITS];
   __shared__ uint32_t wtot[SB_WAVES];
   __shared__ float red[SB_WAVES][6];
   __shared__ float sroot[6];
@@ -135,7 +139,7 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
       mx[c] = fmaxf(mx[c], __shfl_xor(mx[c], off, 64));
     }
   if (lane == 0) put6(red[w], mn, mx);
-  (&hist[0][0])[tid] = 0;                  // 4 x 256 == SB_THREADS
+  for (uint32_t q = tid; q < SB_PASSES * SB_DIGITS; q += SB_THREADS) (&hist[0][0])[q] = 0;
   __syncthreads();
   if (tid < 6) {
     float v = red[0][tid];
@@ -165,13 +169,14 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
       const uint32_t c = box_morton(bxs[u], bxs[u] + 3, rmin, rext);
       kv0[t0 + t] = make_uint2(c, t);
 #pragma unroll
-      for (int p = 0; p < 4; ++p) atomicAdd(&hist[p][(c >> (8 * p)) & 255u], 1u);
+      for (int p = 0; p < SB_PASSES; ++p) atomicAdd(&hist[p][(c >> (SB_BITS * p)) & (SB_DIGITS - 1)], 1u);
     }
   }
   __syncthreads();
-  {  // exclusive scans: thread (p, digit) = (tid / 256, tid % 256)
-    const uint32_t p = tid >> 8, dg = tid & 255u;
-    const uint32_t v = hist[p][dg];
+  // exclusive scans of the counts, one pass at a time: thread = digit
+  static_assert(SB_DIGITS == SB_THREADS, "one digit per thread in the scans");
+  for (int p = 0; p < SB_PASSES; ++p) {
+    const uint32_t v = hist[p][tid];
     uint32_t incl = v;
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t u = __shfl_up(incl, off, 64);
@@ -180,19 +185,19 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
     if (lane == 63) wtot[w] = incl;
     __syncthreads();
     uint32_t base = 0;
-    for (uint32_t q = w & ~3u; q < w; ++q) base += wtot[q];
-    hist[p][dg] = base + incl - v;
+    for (uint32_t q = 0; q < w; ++q) base += wtot[q];
+    hist[p][tid] = base + incl - v;
     __syncthreads();
   }
 
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int p = 0; p < 4; ++p) {
+  for (int p = 0; p < SB_PASSES; ++p) {
     const uint2* src = ((p & 1) ? kv1 : kv0) + t0;
     uint2* dst = ((p & 1) ? kv0 : kv1) + t0;
-    const int shift = 8 * p;
-    if (tid < 256) run[tid] = hist[p][tid];
+    const int shift = SB_BITS * p;
+    run[tid] = hist[p][tid];
     for (uint32_t base = 0; base < m; base += SB_TILE) {
-      for (uint32_t q = lane; q < 256; q += 64) wcnt[w][q] = 0;
+      for (uint32_t q = lane; q < (uint32_t)SB_DIGITS; q += 64) wcnt[w][q] = 0;
       uint2 e[SB_ROWS];
       uint32_t dg[SB_ROWS], loc[SB_ROWS];
       bool ok[SB_ROWS];
@@ -204,10 +209,10 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
       }
 #pragma unroll
       for (int r = 0; r < SB_ROWS; ++r) {
-        dg[r] = (e[r].x >> shift) & 255u;
+        dg[r] = (e[r].x >> shift) & (SB_DIGITS - 1);
         uint64_t msk = __ballot(ok[r]);
 #pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
+        for (int bit = 0; bit < SB_BITS; ++bit) {
           const bool on = (dg[r] >> bit) & 1u;
           const uint64_t bb = __ballot(on);
           msk &= on ? bb : ~bb;
@@ -221,7 +226,7 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
         if (ok[r] && rank == 0) wcnt[w][dg[r]] = prior + (uint32_t)__popcll(msk);
       }
       lds_barrier();
-      if (tid < 256) {               // per digit: offsets of the waves' runs
+      {                                // per digit: offsets of the waves' runs
         uint32_t s = run[tid];
         for (int q = 0; q < SB_WAVES; ++q) {
           const uint32_t c = wcnt[q][tid];
@@ -240,12 +245,13 @@ __device__ inline void bb_sort(uint32_t j, const BodyDesc* __restrict__ desc, co
     }
     __syncthreads();                   // the pass's stores are visible to the next pass's loads
   }
-  // four passes end in kv0: keys (morton << 32 | primitive) and leaves
+  // the passes end in kv1 (odd count) or kv0: keys (morton << 32 | primitive) and leaves
+  const uint2* sorted = ((SB_PASSES & 1) ? kv1 : kv0) + t0;
   for (uint32_t base = 0; base < m; base += SB_BATCH * SB_THREADS) {
     uint2 e[SB_BATCH];
     uint32_t tv[SB_BATCH][3];
 #pragma unroll
-    for (int u = 0; u < SB_BATCH; ++u) e[u] = kv0[t0 + min(base + u * SB_THREADS + tid, m - 1)];
+    for (int u = 0; u < SB_BATCH; ++u) e[u] = sorted[min(base + u * SB_THREADS + tid, m - 1)];
 #pragma unroll
     for (int u = 0; u < SB_BATCH; ++u) {
       const uint32_t* t = tris + 3 * ((size_t)t0 + min(e[u].y, m - 1));
