@@ -100,7 +100,7 @@ struct Lane {
   DBuf<float> partial;              // [6*nblk_box] per-block root boxes
   DBuf<uint32_t> code[2], idx[2];   // radix sort ping-pong
   DBuf<uint32_t> hist;              // [256 * sort blocks]
-  DBuf<uint32_t> digit_total;       // [256] keys per digit of the current pass
+  DBuf<uint32_t> digit_total;       // [1024] keys per digit of the current pass
   DBuf<uint64_t> keys;              // [m] sorted morton << 32 | primitive
   DBuf<int32_t> leaf_parent;        // [m]
   // refit description: internal nodes whose leaf range crosses a chunk

@@ -142,20 +142,28 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ boxes,
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 4;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+// three passes of 10-bit digits cover the 30-bit Morton codes
+constexpr int RADIX_BITS = 10;
+constexpr int RADIX_DIGITS = 1 << RADIX_BITS;
+constexpr int RADIX_PASSES = 3;
+constexpr int DIGITS_PER_THREAD = RADIX_DIGITS / SORT_THREADS;
 
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                              int shift, uint32_t* __restrict__ hist,
                                                              uint32_t nblk) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  __shared__ uint32_t h[RADIX_DIGITS];
+  for (int q = 0; q < DIGITS_PER_THREAD; ++q) h[q * SORT_THREADS + threadIdx.x] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * SORT_TILE;
   for (int r = 0; r < SORT_ITEMS; ++r) {
     const uint32_t e = base + r * SORT_THREADS + threadIdx.x;
-    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & (RADIX_DIGITS - 1)], 1u);
   }
   __syncthreads();
-  hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  for (int q = 0; q < DIGITS_PER_THREAD; ++q) {
+    const uint32_t d = q * SORT_THREADS + threadIdx.x;
+    hist[(size_t)d * nblk + blockIdx.x] = h[d];
+  }
 }
 
 // Per digit, the exclusive prefix over blocks of the block counts (in place)
@@ -180,43 +188,51 @@ __global__ __launch_bounds__(256) void k_radix_rowscan(uint32_t* __restrict__ hi
 }
 
 // Scatter with stable ranks: within a wave, lanes holding the same digit are
-// matched with 8 ballots; waves are ordered through per-digit LDS counts.
+// matched with RADIX_BITS ballots; waves are ordered through per-digit LDS
+// counts.  Thread t owns digits t, t + 256, t + 512, t + 768 in the scans.
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin, uint32_t n,
                                                                 int shift, const uint32_t* __restrict__ hist,
                                                                 const uint32_t* __restrict__ total,
                                                                 uint32_t nblk, uint32_t* __restrict__ kout,
                                                                 uint32_t* __restrict__ vout) {
-  __shared__ uint32_t goff[256];
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[SORT_THREADS / 64][256];
+  __shared__ uint32_t goff[RADIX_DIGITS];
+  __shared__ uint32_t run[RADIX_DIGITS];
+  __shared__ uint32_t wsum[SORT_THREADS / 64];
+  __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX_DIGITS];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  {  // digit bases: exclusive scan of the digit totals (one wave per 64 digits)
-    const uint32_t v = total[tid];
-    uint32_t incl = v;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t u = __shfl_up(incl, off, 64);
-      if (lane >= (uint32_t)off) incl += u;
+  {  // digit bases: exclusive scan of the digit totals, in rounds of 256 digits
+    uint32_t carry = 0;
+    for (int q = 0; q < DIGITS_PER_THREAD; ++q) {
+      const uint32_t d = q * SORT_THREADS + tid;
+      const uint32_t v = total[d];
+      uint32_t incl = v;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += u;
+      }
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      uint32_t base = carry;
+      for (uint32_t k = 0; k < w; ++k) base += wsum[k];
+      goff[d] = base + incl - v + hist[(size_t)d * nblk + blockIdx.x];
+      for (uint32_t k = 0; k < SORT_THREADS / 64; ++k) carry += wsum[k];
+      __syncthreads();
     }
-    run[tid] = incl;                    // wave-inclusive; combined below
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t q = 0; q < w; ++q) base += run[q * 64 + 63];
-    goff[tid] = base + incl - v + hist[(size_t)tid * nblk + blockIdx.x];
-    __syncthreads();
   }
-  run[tid] = 0;
+  for (int q = 0; q < DIGITS_PER_THREAD; ++q) run[q * SORT_THREADS + tid] = 0;
   const uint32_t base = blockIdx.x * SORT_TILE;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    for (int q = 0; q < SORT_THREADS / 64; ++q) wcnt[q][tid] = 0;
+    for (int q = 0; q < SORT_THREADS / 64; ++q)
+      for (int k = 0; k < DIGITS_PER_THREAD; ++k) wcnt[q][k * SORT_THREADS + tid] = 0;
     __syncthreads();
     const uint32_t e = base + r * SORT_THREADS + tid;
     const bool valid = e < n;
     const uint32_t key = valid ? kin[e] : 0u, val = valid ? vin[e] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t d = (key >> shift) & (RADIX_DIGITS - 1);
     uint64_t msk = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < RADIX_BITS; ++b) {
       const uint32_t bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       msk &= bit ? bb : ~bb;
@@ -224,14 +240,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
     const uint32_t rank = __popcll(msk & lt);
     if (valid && rank == 0) wcnt[w][d] = __popcll(msk);
     __syncthreads();
-    {
-      uint32_t s = run[tid];
+    for (int k = 0; k < DIGITS_PER_THREAD; ++k) {
+      const uint32_t dd = k * SORT_THREADS + tid;
+      uint32_t s = run[dd];
       for (int q = 0; q < SORT_THREADS / 64; ++q) {
-        const uint32_t c = wcnt[q][tid];
-        wcnt[q][tid] = s;
+        const uint32_t c = wcnt[q][dd];
+        wcnt[q][dd] = s;
         s += c;
       }
-      run[tid] = s;
+      run[dd] = s;
     }
     __syncthreads();
     if (valid) {
@@ -294,8 +311,8 @@ void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
     code[b].alloc(m);
     idx[b].alloc(m);
   }
-  hist.alloc(256 * (size_t)grid_for(m, SORT_TILE));
-  digit_total.alloc(256);
+  hist.alloc(RADIX_DIGITS * (size_t)grid_for(m, SORT_TILE));
+  digit_total.alloc(RADIX_DIGITS);
   if (bottom_up) {
     const size_t c0 = level_chunks0(m);
     for (int q = 0; q < 2; ++q) {
@@ -406,9 +423,9 @@ void Body::build(int k, int l, hipStream_t s) {
   CC_HIP(hipGetLastError());
   const uint32_t nblk = grid_for(m, SORT_TILE);
   int src = 0;
-  for (int shift = 0; shift < 32; shift += 8) {
+  for (int pass = 0, shift = 0; pass < RADIX_PASSES; ++pass, shift += RADIX_BITS) {
     k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, m, shift, L.hist.p, nblk);
-    k_radix_rowscan<<<64, 256, 0, s>>>(L.hist.p, nblk, L.digit_total.p);
+    k_radix_rowscan<<<RADIX_DIGITS / 4, 256, 0, s>>>(L.hist.p, nblk, L.digit_total.p);
     k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p,
                                                   L.digit_total.p, nblk, L.code[src ^ 1].p, L.idx[src ^ 1].p);
     CC_HIP(hipGetLastError());
