@@ -236,11 +236,15 @@ int cc_sim_destroy(cc_sim* sim);
    the fused Verlet + spring + collision kernel (verlet.cu:129-161) against
    the body pose of that frame, then the ping-pong swap (:243-251).  A
    REFIT / REBUILD body's poses and LBVHs are computed on the library's own
-   streams, up to 16 frames ahead of the cloth.  Asynchronous on stream
-   (NULL: the simulator's own stream); runs of 16 steps are replayed from a
-   captured hipGraph. */
+   streams, up to one block (cc_sim_block_steps, 24 by default) ahead of the
+   cloth.  Asynchronous on stream (NULL: the simulator's own stream); whole
+   blocks are replayed from captured hipGraphs, the remaining steps are
+   launched one by one. */
 int cc_sim_step(cc_sim* sim, int nsteps, void* stream);
 int cc_sim_synchronize(cc_sim* sim);
+/* Steps per captured block (even).  No counterpart in the reference, whose
+   loop launches every step from the host. */
+int cc_sim_block_steps(cc_sim* sim, int* steps);
 
 /* Simulator::update_vertex (simulation.cu:253-257): x_cur = x_last = xyz. */
 int cc_sim_update_vertex(cc_sim* sim, uint32_t idx, const float xyz[3]);

@@ -112,6 +112,7 @@ PROTOTYPES = {
     "cc_sim_get_state": (C.c_int, [P, P, P, P]),
     "cc_sim_set_state": (C.c_int, [P, P, P, P]),
     "cc_sim_frame": (C.c_int, [P, C.POINTER(u64)]),
+    "cc_sim_block_steps": (C.c_int, [P, C.POINTER(C.c_int)]),
     "cc_sim_set_frame": (C.c_int, [P, u64]),
     "cc_sim_set_debug": (C.c_int, [P, C.c_int]),
     "cc_sim_get_debug": (C.c_int, [P, P, P, P]),

@@ -15,7 +15,7 @@
 namespace cc {
 
 constexpr int MAX_CAPS = 64;
-constexpr int MAX_SLOTS = 32;
+constexpr int MAX_SLOTS = 64;
 constexpr int MAX_LANES = 4;
 
 // Per-frame motion of a capsule-union body (see cc_body_desc), by value.

@@ -59,9 +59,13 @@ struct Sim {
   DBuf<uint32_t> faces, vrow, vcol;
   DBuf<float> face_n, vnormals;
 
-  // Graphs of GRAPH_STEPS steps, per slot of the first frame and state
+  // Graphs of graph_steps steps, per slot of the first frame and state
   // parity (captured when first needed).
-  static constexpr int GRAPH_STEPS = 16;
+  // Steps per block (even; a moving body's ring holds 2 * graph_steps
+  // poses).  24 = 8 updates on each of three lanes: at the 501k-triangle
+  // refitted body, 52.4 us per step against 54.5 with 16 (6/5/5 updates:
+  // the step stream waits for the lane with six) and 53.8 with 32.
+  int graph_steps = 24;
   hipGraphExec_t gmain[MAX_SLOTS][2] = {};
   // update lanes (moving bodies): streams and the events that order them
   // against the step stream
@@ -153,11 +157,11 @@ struct Sim {
     return end_capture(cur0);
   }
 
-  // GRAPH_STEPS fused steps from a frame in slot k0, starting at the current
+  // graph_steps fused steps from a frame in slot k0, starting at the current
   // state parity.
   hipGraphExec_t capture_main(int k0) {
     return capture([&] {
-      for (int j = 0; j < GRAPH_STEPS; ++j) {
+      for (int j = 0; j < graph_steps; ++j) {
         launch_verlet(body.slot_of((uint64_t)(k0 + j)), stream);
         cur ^= 1;
       }
@@ -168,7 +172,7 @@ struct Sim {
   void collect(int set) {
     if (!tpending[set]) return;
     hipEvent_t* te = tev[set];
-    const int G = GRAPH_STEPS;
+    const int G = graph_steps;
     CC_HIP(hipEventSynchronize(te[1]));
     float ms = 0;
     CC_HIP(hipEventElapsedTime(&ms, te[0], te[1]));
@@ -202,20 +206,20 @@ struct Sim {
     }
   }
 
-  // One block of GRAPH_STEPS steps.  A moving body's poses for these frames
+  // One block of graph_steps steps.  A moving body's poses for these frames
   // are in their slots (the previous block's lanes computed them, or they
   // are computed here first); the lanes compute the poses of the next
-  // GRAPH_STEPS frames, into the slots of the last GRAPH_STEPS frames (the
-  // ring holds 2 * GRAPH_STEPS), while the steps run:
+  // graph_steps frames, into the slots of the last graph_steps frames (the
+  // ring holds 2 * graph_steps), while the steps run:
   //
-  //   step stream:  join(lanes of last block) -> go -> [step graph: 16 fused steps]
-  //   lane l:       wait(go) -> U(f+16+l), U(f+16+l+L), ... -> done[l]
+  //   step stream:  join(lanes of last block) -> go -> [step graph: G fused steps]
+  //   lane l:       wait(go) -> U(f+G+l), U(f+G+l+L), ... -> done[l]
   //
   // A dependency between streams costs several microseconds, so a block has
   // two, not two per step.  "go" follows the previous step graph, so a lane
   // overwrites a slot only after the steps that read it.
   void step_graph(hipStream_t s) {
-    const int G = GRAPH_STEPS;
+    const int G = graph_steps;
     const int k0 = body.slot_of(frame);
     hipEvent_t* te = timing ? tev[tset] : nullptr;
     if (timing) collect(tset);
@@ -257,15 +261,15 @@ struct Sim {
     frame += (uint64_t)G;   // G is even: the state parity is unchanged
   }
 
-  // Runs of GRAPH_STEPS steps go through the graphs; the rest are launched
+  // Runs of graph_steps steps go through the graphs; the rest are launched
   // one by one, each followed by the body update of the next frame when it
   // is not ready yet.
   void step(int nsteps, hipStream_t s) {
     if (!s) s = stream;
     while (nsteps > 0) {
-      if (nsteps >= GRAPH_STEPS) {
+      if (nsteps >= graph_steps) {
         step_graph(s);
-        nsteps -= GRAPH_STEPS;
+        nsteps -= graph_steps;
       } else {
         step_single(s);
         --nsteps;

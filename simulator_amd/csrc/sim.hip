@@ -343,8 +343,11 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       int lanes = 3;
       if (const char* e = std::getenv("CC_BODY_LANES")) lanes = std::atoi(e);
       CC_REQUIRE(lanes >= 1 && lanes <= MAX_LANES, "cc_sim_create: CC_BODY_LANES must be 1..4");
+      if (const char* e = std::getenv("CC_GRAPH_STEPS")) S.graph_steps = std::atoi(e);
+      CC_REQUIRE(S.graph_steps >= 2 && S.graph_steps % 2 == 0 && 2 * S.graph_steps <= MAX_SLOTS,
+                 "cc_sim_create: CC_GRAPH_STEPS must be even, 2..MAX_SLOTS/2");
       S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, mp,
-                  2 * Sim::GRAPH_STEPS, lanes, s);
+                  2 * S.graph_steps, lanes, s);
       // frame 0 into slot 0; a refitted body keeps the topology built from
       // the frame-0 pose in every slot
       S.body.deform(0, 0, s);
@@ -427,6 +430,13 @@ int cc_sim_frame(cc_sim* sim, uint64_t* frame) {
   return guard([&] {
     CC_REQUIRE(sim && frame, "cc_sim_frame: null");
     *frame = sim->s.frame;
+  });
+}
+
+int cc_sim_block_steps(cc_sim* sim, int* steps) {
+  return guard([&] {
+    CC_REQUIRE(sim && steps, "cc_sim_block_steps: null");
+    *steps = sim->s.graph_steps;
   });
 }
 

@@ -273,6 +273,15 @@ class Simulator:
         L.check(L.lib().cc_sim_frame(self._h, C.byref(f)))
         return f.value
 
+    @property
+    def block_steps(self):
+        """Steps per captured block: simulate() replays whole blocks from
+        graphs (a moving body's poses computed a block ahead on the update
+        lanes) and launches the remaining steps one by one."""
+        g = C.c_int()
+        L.check(L.lib().cc_sim_block_steps(self._h, C.byref(g)))
+        return g.value
+
     def set_frame(self, frame):
         L.check(L.lib().cc_sim_set_frame(self._h, int(frame)))
 

@@ -155,16 +155,18 @@ def test_step_collides_with_the_pose_of_its_own_frame(gpu, mode):
 @pytest.mark.parametrize("mode", [L.CC_BODY_REFIT, L.CC_BODY_REBUILD])
 def test_multi_step_graph_equals_single_steps(gpu, mode):
     sc = moving_scene(mode)
-    say(f"[body] 3 + 45 steps (graphs of 16 from frame 3) vs 48 single steps (mode {mode})")
     a, b = Simulator.from_scene(sc), Simulator.from_scene(sc)
+    G = a.block_steps
+    total = 3 + 2 * G + 5
+    say(f"[body] 3 + {total - 3} steps (graphs of {G} from frame 3) vs {total} single steps (mode {mode})")
     a.simulate(3)
-    a.simulate(45)
-    for _ in range(48):
+    a.simulate(total - 3)
+    for _ in range(total):
         b.simulate(1)
     for x, y in zip(a.get_state(), b.get_state()):
         np.testing.assert_array_equal(x, y)
     np.testing.assert_array_equal(a.body_vertices(), b.body_vertices())
-    assert a.frame == b.frame == 48
+    assert a.frame == b.frame == total
     a.close()
     b.close()
 
@@ -172,16 +174,17 @@ def test_multi_step_graph_equals_single_steps(gpu, mode):
 def test_in_graph_timing_reports_every_graph_step(gpu):
     sc = moving_scene(L.CC_BODY_REFIT, side=32)
     sim = Simulator.from_scene(sc)
+    G = sim.block_steps
     sim.set_timing(True)
-    sim.simulate(5 * 16 + 3)          # five graphs, then three single steps
+    sim.simulate(5 * G + 3)           # five graphs, then three single steps
     t = sim.timing()
     say(f"[body] in-graph timing: {t}")
-    assert t["steps"] == 5 * 16
+    assert t["steps"] == 5 * G
     assert t["verlet_us"] > 0 and t["refit_us"] > 0
     x = sim.get_state()[0]
     sim.set_timing(False)
     ref = Simulator.from_scene(sc)
-    ref.simulate(5 * 16 + 3)
+    ref.simulate(5 * G + 3)
     np.testing.assert_array_equal(ref.get_state()[0], x)     # timing changes no result
     sim.close()
     ref.close()
