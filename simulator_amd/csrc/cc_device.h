@@ -159,6 +159,10 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
     const float4* nd = b.wide + 8 * (size_t)node;
     const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6],
                  q7 = nd[7];
+    // All four links arrive with the boxes: otherwise the compiler loads
+    // entry 0's link apart, inside the branch that reads it, one more
+    // memory round trip per descending visit.
+    asm volatile("" ::"v"(q1.z), "v"(q3.z), "v"(q5.z), "v"(q7.z));
     ++visits;
     r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, st, node);
   }
