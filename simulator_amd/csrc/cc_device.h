@@ -14,9 +14,18 @@ namespace cc {
 // STACK_SPILL entries per lane: a private array here would make the kernel
 // use scratch memory, which was seen to cap the number of resident waves.
 constexpr int STACK_SPILL = 60;
+// The four-wide walk keeps its first WSTK_LDS deferred entries in LDS, entry
+// k of a lane at lds[k * WSTK_LANES] (the kernel's workgroup size), so every
+// lane of a wave hits its own bank whatever its depth; deeper entries go to
+// the global spill area.  16 entries (16 KB per workgroup) hold the deepest
+// stacks the walks reach at the large workload (9), and leave room in a CU
+// for the body-update kernels that run beside the step.
+constexpr int WSTK_LDS = 16;
+constexpr int WSTK_LANES = 256;
 struct Spill {
   int32_t* p;
   uint32_t stride;
+  int32_t* lds = nullptr;   // this lane's entry 0 (kernels that walk the four-wide form)
 };
 
 // Deferred nodes: the top four in registers (static shifts, no indexed
@@ -123,38 +132,71 @@ __device__ inline WideRoot load_wide_root(const BvhView& b) {
 // The set-up keeps the wide form only when no root path defers more than
 // 64 entries (sum of entries - 1 over the path), so the stack cannot
 // overflow.
-// One visit of a wide record q[0..7] (entry e = q[2e], q[2e + 1]): >= 0 the
-// hit leaf, WIDE_NEXT with the next record in `node`, WIDE_MISS.
+__device__ inline void wstk_put(const Spill& sp, int k, int32_t v) {
+  if (k < WSTK_LDS)
+    sp.lds[k * WSTK_LANES] = v;
+  else
+    sp.p[(size_t)(k - WSTK_LDS) * sp.stride] = v;
+}
+__device__ inline int32_t wstk_get(const Spill& sp, int k) {
+  return k < WSTK_LDS ? sp.lds[k * WSTK_LANES] : sp.p[(size_t)(k - WSTK_LDS) * sp.stride];
+}
+
+// One visit of a wide record q[0..7] (entry e is q[2e], q[2e + 1]) takes
+// the first entry whose box holds p: a leaf is the hit (returned, >= 0), an
+// internal entry the next record (WIDE_NEXT, in `node`); with none, the
+// walk pops (a deferred leaf is the hit) or misses (WIDE_MISS).  The later
+// candidates of a visit (at most three) are deferred with three
+// unconditional stores at the stack top, highest entry first, and the stack
+// pointer then moves by their number: a slot above the new top holds
+// nothing the walk reads.  This takes far fewer instructions than shifting
+// a register stack entry by entry, which matters because all four waves of
+// a SIMD walk at once.
 enum : int { WIDE_NEXT = -1, WIDE_MISS = -2 };
 __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4, float4 q5,
-                                     float4 q6, float4 q7, TravStack& st, int32_t& node) {
-  uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
-                  (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
-                  (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
-                  (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
+                                     float4 q6, float4 q7, const Spill& spill, int& sp, int32_t& node) {
+  const uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
+                        (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
+                        (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
+                        (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
   const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
                 c3 = __float_as_int(q7.z);
   if (mask == 0) {
-    if (!st.pop(node)) return WIDE_MISS;
+    if (sp == 0) return WIDE_MISS;
+    --sp;
+    node = wstk_get(spill, sp);
     return node < 0 ? ~node : WIDE_NEXT;
   }
   const uint32_t first = __builtin_ctz(mask);
   const int32_t c = first == 0 ? c0 : first == 1 ? c1 : first == 2 ? c2 : c3;
   if (c < 0) return ~c;
-  mask &= mask - 1;
-  if ((mask & 8u) && !st.push(c3)) return WIDE_MISS;
-  if ((mask & 4u) && !st.push(c2)) return WIDE_MISS;
-  if ((mask & 2u) && !st.push(c1)) return WIDE_MISS;
+  // the later candidates, popped lowest entry first
+  const uint32_t rest = mask & (mask - 1);
+  const bool b2 = (rest & 4u) != 0, b3 = (rest & 8u) != 0;
+  const int32_t x0 = b3 ? c3 : (b2 ? c2 : c1);
+  const int32_t x1 = b3 ? (b2 ? c2 : c1) : c1;
+  const int k = __builtin_popcount(rest);
+  if (sp + k > 64) return WIDE_MISS;    // unreachable: the set-up bounds the deferrals by 64
+  if (sp + 3 <= WSTK_LDS) {
+    int32_t* t = spill.lds + sp * WSTK_LANES;
+    t[0] = x0;
+    t[WSTK_LANES] = x1;
+    t[2 * WSTK_LANES] = c1;
+  } else {
+    if (k > 0) wstk_put(spill, sp, x0);
+    if (k > 1) wstk_put(spill, sp + 1, x1);
+    if (k > 2) wstk_put(spill, sp + 2, c1);
+  }
+  sp += k;
   node = c;
   return WIDE_NEXT;
 }
 
 __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill, const WideRoot& rt) {
-  TravStack st;
-  st.spill = spill;
+  int sp = 0;
   int32_t node = 0;
   ++visits;
-  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], st, node);
+  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node);
   for (uint32_t guard = 0; r == WIDE_NEXT && guard < b.m; ++guard) {
     const float4* nd = b.wide + 8 * (size_t)node;
     const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6],
@@ -164,7 +206,7 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
     // memory round trip per descending visit.
     asm volatile("" ::"v"(q1.z), "v"(q3.z), "v"(q5.z), "v"(q7.z));
     ++visits;
-    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, st, node);
+    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, spill, sp, node);
   }
   return r >= 0 ? r : -1;
 }

@@ -22,7 +22,10 @@ namespace cc {
 // One thread per vertex.  Spring forces use FMA and rsqrt (positions are
 // checked against the FP64 oracle by tolerance); the collision part is exact
 // order so the contact set is bit-exact for the pre-projection positions.
-template <bool DEBUG, bool COUNT, int EW>
+// WIDE: the body is walked through its four-wide form (the walk's stack in
+// LDS); otherwise the kernel uses no LDS, which leaves the CUs' LDS to the
+// body-update kernels that run beside it.
+template <bool DEBUG, bool COUNT, int EW, bool WIDE>
 __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, float* __restrict__ sout,
                                                 const float* contact_in, float* contact_out, const uint32_t* __restrict__ row,
                                                 const uint2* __restrict__ rec, StepArgs a, BvhView bvh,
@@ -33,8 +36,13 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
   uint32_t visits = 0;
   const bool active = i < n;
   if (active) {
-    step_vertex<DEBUG, EW, true>(i, sin_, sout, contact_in, contact_out, row, rec, a, bvh, Spill{spill + i, n}, visits, dbg_pre,
-                            dbg_hit, dbg_in);
+    int32_t* lds = nullptr;
+    if constexpr (WIDE) {
+      __shared__ int32_t wstk[WSTK_LDS * WSTK_LANES];
+      lds = wstk + threadIdx.x;
+    }
+    step_vertex<DEBUG, EW, WIDE>(i, sin_, sout, contact_in, contact_out, row, rec, a, bvh,
+                                 Spill{spill + i, n, lds}, visits, dbg_pre, dbg_hit, dbg_in);
   }
   if (COUNT) {
     // block sum into this block's own counter pair (no shared atomics)
@@ -61,11 +69,12 @@ __global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float
                                                         uint8_t* __restrict__ ins, float* __restrict__ out_pos,
                                                         float* __restrict__ out_n) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  __shared__ int32_t wstk[WSTK_LDS * WSTK_LANES];
   if (i >= n) return;
   f3 p = ld3(pts + 3 * (size_t)i), nc;
   int32_t h;
   uint32_t visits = 0;
-  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n}, load_wide_root(bvh));
+  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n, wstk + threadIdx.x}, load_wide_root(bvh));
   hit[i] = h;
   ins[i] = in ? 1 : 0;
   st3(out_pos + 3 * (size_t)i, p);
@@ -129,15 +138,27 @@ template <bool DEBUG, bool COUNT>
 static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, const float* contact_in, float* contact_out,
                             const StepArgs& a, const BvhView& v, float* dbg_pre, int32_t* dbg_hit, uint8_t* dbg_in,
                             unsigned long long* ctr) {
-  if (S.ell_w == ELL_W)
-    k_verlet<DEBUG, COUNT, ELL_W><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
-                                                    dbg_pre, dbg_hit, dbg_in, ctr);
-  else if (S.ell_w == ELL_W2)
-    k_verlet<DEBUG, COUNT, ELL_W2><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v, S.n, S.spill.p,
-                                                     dbg_pre, dbg_hit, dbg_in, ctr);
-  else
-    k_verlet<DEBUG, COUNT, 0><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n, S.spill.p,
-                                                dbg_pre, dbg_hit, dbg_in, ctr);
+  if (v.wide) {
+    if (S.ell_w == ELL_W)
+      k_verlet<DEBUG, COUNT, ELL_W, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
+                                                            S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+    else if (S.ell_w == ELL_W2)
+      k_verlet<DEBUG, COUNT, ELL_W2, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
+                                                             S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+    else
+      k_verlet<DEBUG, COUNT, 0, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n,
+                                                        S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+  } else {
+    if (S.ell_w == ELL_W)
+      k_verlet<DEBUG, COUNT, ELL_W, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
+                                                             S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+    else if (S.ell_w == ELL_W2)
+      k_verlet<DEBUG, COUNT, ELL_W2, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
+                                                              S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+    else
+      k_verlet<DEBUG, COUNT, 0, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n,
+                                                         S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+  }
   CC_HIP(hipGetLastError());
 }
 
