@@ -83,3 +83,49 @@ def test_width_switch_and_rebuild(gpu):
     g = Simulator.from_scene(S.garment(side=16))
     assert g.bvh_width()[0] == 2    # rebuilt every frame: always the binary walk
     g.close()
+
+
+def crossed_body(pairs):
+    """2 * pairs + 1 triangles whose boxes share the centre (0, 0, 0), so one
+    Morton code and a tree interleaved by index: per pair one triangle flat in
+    y and one flat in x, then one whose box is the cube [-1, 1]^3.  A point
+    such as (0.5, 0.5, z) is in the box of every node that holds both kinds
+    and in no leaf box but the last one's: the walk visits the whole tree,
+    deferring three entries at most levels.  With 3000 pairs (6,001 leaves)
+    the last leaf lies at the bottom of the rightmost path; an emulation of
+    the four-wide walk over the oracle's tree visits 3,393 records for such a
+    point, with up to 18 entries deferred at once.  (With 2^k + 1 leaves the
+    last one would be the root's own leaf child, found at once.)"""
+    v = []
+    for _ in range(pairs):
+        v += [(-1.0, -0.1, -1.0), (1.0, 0.1, -1.0), (0.0, 0.0, 1.0)]
+        v += [(-0.1, -1.0, -1.0), (0.1, 1.0, -1.0), (0.0, 0.0, 1.0)]
+    v += [(-1.0, -1.0, -1.0), (1.0, 1.0, -1.0), (0.0, 0.0, 1.0)]
+    verts = np.array(v, np.float32)
+    tris = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    return S.Body(verts, tris, np.zeros_like(verts), L.CC_BODY_STATIC, False)
+
+
+def test_queries_deep_stack_wide_equals_binary(gpu):
+    # the box depth is 0 so that a box's centre is its triangle's centroid
+    sc = S.grid_scene("crossed", 8, 1e-3, 10, crossed_body(3000), box_depth=0.0)
+    a, b = pair(sc)
+    w, nw, bound = a.bvh_width()
+    # more deferrals than the LDS part of the stack holds (16 entries)
+    assert w == 4 and 16 < bound <= 64, (w, nw, bound)
+    rng = np.random.default_rng(11)
+    z = rng.uniform(-0.9, 0.9, 96)
+    s = rng.choice([-0.5, 0.5], size=(96, 2))
+    deep = np.stack([s[:, 0], s[:, 1], z], axis=1)            # the whole tree, then the last leaf
+    near = np.stack([rng.uniform(-0.9, 0.9, 64), rng.uniform(-0.05, 0.05, 64),
+                     rng.uniform(-0.9, 0.9, 64)], axis=1)     # in the flat-in-y boxes
+    out = np.array([[2.0, 0.0, 0.0], [0.0, 0.0, 1.5], [-1.5, 0.5, 0.0]])
+    pts = np.concatenate([deep, near, out]).astype(np.float32)
+    say(f"[wide] crossed body: {nw} wide nodes, deferral bound {bound}, {pts.shape[0]} queries")
+    ra, rb = a.collide_points(pts), b.collide_points(pts)
+    for x, y in zip(ra, rb):
+        np.testing.assert_array_equal(x, y)
+    assert np.all(ra[0][:96] == ra[0][0]) and ra[0][0] >= 0    # every deep query ends on the same leaf
+    assert np.all(ra[0][-3:] == -1)
+    a.close()
+    b.close()
