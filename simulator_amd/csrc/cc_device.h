@@ -138,8 +138,14 @@ __device__ inline void wstk_put(const Spill& sp, int k, int32_t v) {
   else
     sp.p[(size_t)(k - WSTK_LDS) * sp.stride] = v;
 }
+// The LDS read is unconditional (index clamped) and the global one only on
+// the rare deep branch: a select between the two addresses would become one
+// generic (flat) load.
 __device__ inline int32_t wstk_get(const Spill& sp, int k) {
-  return k < WSTK_LDS ? sp.lds[k * WSTK_LANES] : sp.p[(size_t)(k - WSTK_LDS) * sp.stride];
+  int32_t v = sp.lds[min(k, WSTK_LDS - 1) * WSTK_LANES];
+  asm volatile("" : "+v"(v));
+  if (k >= WSTK_LDS) v = sp.p[(size_t)(k - WSTK_LDS) * sp.stride];
+  return v;
 }
 
 // One visit of a wide record q[0..7] (entry e is q[2e], q[2e + 1]) takes
@@ -167,8 +173,11 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
     node = wstk_get(spill, sp);
     return node < 0 ? ~node : WIDE_NEXT;
   }
-  const uint32_t first = __builtin_ctz(mask);
-  const int32_t c = first == 0 ? c0 : first == 1 ? c1 : first == 2 ? c2 : c3;
+  // the link of the lowest set bit (selects, not a branch per case)
+  int32_t c = c3;
+  c = (mask & 4u) ? c2 : c;
+  c = (mask & 2u) ? c1 : c;
+  c = (mask & 1u) ? c0 : c;
   if (c < 0) return ~c;
   // the later candidates, popped lowest entry first
   const uint32_t rest = mask & (mask - 1);
