@@ -161,30 +161,27 @@ __device__ inline int32_t wstk_get(const Spill& sp, int k) {
 enum : int { WIDE_NEXT = -1, WIDE_MISS = -2 };
 __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4, float4 q5,
                                      float4 q6, float4 q7, const Spill& spill, int& sp, int32_t& node) {
-  const uint32_t mask = (in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y) ? 1u : 0u) |
-                        (in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y) ? 2u : 0u) |
-                        (in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y) ? 4u : 0u) |
-                        (in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y) ? 8u : 0u);
+  // Kept as booleans (lane masks): the first candidate and the deferred set
+  // follow from them by selects, with no per-lane bit mask to build.
+  const bool i0 = in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y), i1 = in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y),
+             i2 = in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y), i3 = in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y);
   const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
                 c3 = __float_as_int(q7.z);
-  if (mask == 0) {
+  if (!(i0 | i1 | i2 | i3)) {
     if (sp == 0) return WIDE_MISS;
     --sp;
     node = wstk_get(spill, sp);
     return node < 0 ? ~node : WIDE_NEXT;
   }
-  // the link of the lowest set bit (selects, not a branch per case)
-  int32_t c = c3;
-  c = (mask & 4u) ? c2 : c;
-  c = (mask & 2u) ? c1 : c;
-  c = (mask & 1u) ? c0 : c;
+  // the first candidate's link
+  const int32_t c = i0 ? c0 : (i1 ? c1 : (i2 ? c2 : c3));
   if (c < 0) return ~c;
-  // the later candidates, popped lowest entry first
-  const uint32_t rest = mask & (mask - 1);
-  const bool b2 = (rest & 4u) != 0, b3 = (rest & 8u) != 0;
-  const int32_t x0 = b3 ? c3 : (b2 ? c2 : c1);
-  const int32_t x1 = b3 ? (b2 ? c2 : c1) : c1;
-  const int k = __builtin_popcount(rest);
+  // the later candidates (a candidate with one before it), popped lowest
+  // entry first
+  const bool r1 = i1 & i0, r2 = i2 & (i0 | i1), r3 = i3 & (i0 | i1 | i2);
+  const int32_t x0 = r3 ? c3 : (r2 ? c2 : c1);
+  const int32_t x1 = r3 ? (r2 ? c2 : c1) : c1;
+  const int k = (int)r1 + (int)r2 + (int)r3;
   if (sp + k > 64) return WIDE_MISS;    // unreachable: the set-up bounds the deferrals by 64
   if (sp + 3 <= WSTK_LDS) {
     int32_t* t = spill.lds + sp * WSTK_LANES;
@@ -207,7 +204,10 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
   ++visits;
   int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node);
   for (uint32_t guard = 0; r == WIDE_NEXT && guard < b.m; ++guard) {
-    const float4* nd = b.wide + 8 * (size_t)node;
+    // a 32-bit byte offset (records < 4 GiB): the loads take the base in
+    // scalar registers
+    const float4* nd = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(b.wide) +
+                                                       ((uint32_t)node << 7));
     const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6],
                  q7 = nd[7];
     // All four links arrive with the boxes: otherwise the compiler loads
