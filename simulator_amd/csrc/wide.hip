@@ -78,18 +78,50 @@ void Body::wide_setup(int k, hipStream_t s) {
       es[cnt] = (uint32_t)rec * 2u + (uint32_t)side;
       ref[cnt++] = child;
     };
-    const int32_t A = nd[x].childA, B = nd[x].childB;
-    if (A < 0) add(x, 0, A);
-    if (B < 0) add(x, 1, B);
-    for (int side = 0; side < 2; ++side) {
-      const int32_t c = side ? B : A;
-      if (c < 0) continue;
+    // X's children in its order of consideration (leaf children first),
+    // then, while there is room, the internal entry with the largest box
+    // surface is replaced in place by its own children in their order: an
+    // in-place replacement keeps the whole list in the binary walk's order
+    // of consideration.  (A CPU replay of a late frame of the large workload:
+    // 7.44 wide visits per query and 12.4 per wave's slowest lane, against
+    // 7.69 and 12.95 when every node folds exactly its children.)
+    add(x, 0, nd[x].childA);
+    add(x, 1, nd[x].childB);
+    auto order_pair = [&](uint32_t e0) {        // leaf child of the pair first
+      if (ref[e0] >= 0 && ref[e0 + 1] < 0) {
+        std::swap(es[e0], es[e0 + 1]);
+        std::swap(ref[e0], ref[e0 + 1]);
+      }
+    };
+    order_pair(0);
+    while (cnt < 4) {
+      int best = -1;
+      double best_area = -1.0;
+      for (uint32_t e = 0; e < cnt; ++e) {
+        if (ref[e] < 0) continue;
+        const Node& P = nd[es[e] >> 1];
+        const float* bmn = (es[e] & 1u) ? P.b_min : P.a_min;
+        const float* bmx = (es[e] & 1u) ? P.b_max : P.a_max;
+        const double dx = (double)bmx[0] - bmn[0], dy = (double)bmx[1] - bmn[1], dz = (double)bmx[2] - bmn[2];
+        const double area = dx * dy + dy * dz + dz * dx;
+        if (area > best_area) {
+          best_area = area;
+          best = (int)e;
+        }
+      }
+      if (best < 0) break;
+      const int32_t c = ref[best];
       CC_REQUIRE((uint32_t)c < mi, "wide form: link out of range");
-      const int32_t ca = nd[c].childA, cb = nd[c].childB;
-      if (ca < 0) add(c, 0, ca);
-      if (cb < 0) add(c, 1, cb);
-      if (ca >= 0) add(c, 0, ca);
-      if (cb >= 0) add(c, 1, cb);
+      for (uint32_t e = cnt; e > (uint32_t)best + 1; --e) {
+        es[e] = es[e - 1];
+        ref[e] = ref[e - 1];
+      }
+      ++cnt;
+      es[best] = (uint32_t)c * 2u;
+      ref[best] = nd[c].childA;
+      es[best + 1] = (uint32_t)c * 2u + 1u;
+      ref[best + 1] = nd[c].childB;
+      order_pair((uint32_t)best);
     }
     const uint32_t deferrable = acc[h] + (cnt - 1);
     worst = std::max(worst, deferrable);
