@@ -109,6 +109,8 @@ struct Lane {
   DBuf<int32_t> top_pos;            // [max(m-1, 1)] node -> position in top_list, -1 if not top
   DBuf<uint32_t> top_count;         // [1]
   DBuf<int4> top_links;             // [max(m-1, 1)] per entry: top child A / B, parent, parent slot
+  DBuf<int4> links;                 // [max(m-1, 1)] per node: childA, childB, parent, other (the
+                                    // refit reads these 16 bytes, not the 64-byte node records)
   DBuf<uint8_t> top_h;              // [max(m-1, 1)] height of each entry in the top tree
   DBuf<uint32_t> top_hmax;          // [1]
   DBuf<uint32_t> top_lev;           // [66] entries of top height h are [top_lev[h], top_lev[h+1])

@@ -329,6 +329,8 @@ void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
   top_count.alloc(1);
   top_count.zero(s);
   top_links.alloc(mi);
+  links.alloc(mi);
+  links.zero(s);
   top_h.alloc(mi);
   top_hmax.alloc(1);
   top_hmax.zero(s);

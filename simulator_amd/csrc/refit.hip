@@ -198,10 +198,18 @@ __global__ __launch_bounds__(TOP_THREADS) void k_top_heights(const int4* __restr
   if (threadIdx.x == 0) *top_hmax = s_hmax;
 }
 
+// The link fields of the node records, compact: a refit reads 16 bytes per
+// node instead of the 128-byte lines that hold its 64-byte record.
+__global__ __launch_bounds__(256) void k_node_links(const Node* __restrict__ nodes, uint32_t mi,
+                                                   int4* __restrict__ links) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < mi) links[i] = reinterpret_cast<const int4*>(nodes + i)[3];
+}
+
 // ======================================================== per-frame refit
 __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     const float* __restrict__ verts, const uint4* __restrict__ leaf_tri, const int32_t* __restrict__ leaf_parent,
-    Node* __restrict__ nodes, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
+    const int4* __restrict__ links, Node* __restrict__ nodes, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
     const uint8_t* __restrict__ lflag, const uint32_t* __restrict__ chunk_h, const int32_t* __restrict__ top_pos,
     float* __restrict__ top_child, float* __restrict__ root_out, float* __restrict__ leaf0_box,
     float4* __restrict__ wide, const uint2* __restrict__ wdst) {
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     tri[r] = leaf_tri[lc];
     lf[r] = lflag[lc];
     lpar[r] = leaf_parent[lc];
-    L[r] = reinterpret_cast<const int4*>(nodes + xc)[3];
+    L[r] = links[xc];
     fl[r] = nflag[xc];
     wd[r] = wide ? reinterpret_cast<const uint4*>(wdst)[xc] : make_uint4(~0u, 0u, ~0u, 0u);
   }
@@ -483,6 +491,8 @@ void Body::top_setup(int k, int l, hipStream_t s) {
   Lane& L = lane[l];
   const unsigned nchunk = grid_for(m, RC_CHUNK);
   const Node* nd = slot[k].nodes.p;
+  k_node_links<<<grid_for(m - 1, 256), 256, 0, s>>>(nd, m - 1, L.links.p);
+  CC_HIP(hipGetLastError());
   k_top_list<<<nchunk, RC_THREADS, 0, s>>>(nd, m, L.top_list.p, L.top_pos.p, L.top_count.p);
   CC_HIP(hipGetLastError());
   k_chunk_heights<<<nchunk, RC_THREADS, 0, s>>>(nd, L.leaf_parent.p, m, L.top_pos.p, L.nflag.p, L.lflag.p,
@@ -515,7 +525,7 @@ void Body::refit(int k, int l, hipStream_t s, bool binary) {
   // the binary boxes instead (export, or the binary walk).
   float4* W = (nwide && wide_enabled && !binary) ? S.wide.p : nullptr;
   k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(S.verts.p, leaf_tri_of(k), T.leaf_parent.p,
-                                                              S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
+                                                              T.links.p, S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
                                                               T.chunk_h.p, T.top_pos.p, L.top_child.p, S.root.p,
                                                               S.leaf0_box.p, W, w_dst.p);
   CC_HIP(hipGetLastError());
