@@ -22,6 +22,8 @@ constexpr int STACK_SPILL = 60;
 // for the body-update kernels that run beside the step.
 constexpr int WSTK_LDS = 16;
 constexpr int WSTK_LANES = 256;
+// the wide walk defers at most 64 entries: the ones past the LDS part fit
+static_assert(64 - WSTK_LDS <= STACK_SPILL, "wide-walk stack beyond LDS exceeds the spill area");
 struct Spill {
   int32_t* p;
   uint32_t stride;
