@@ -47,6 +47,24 @@ def algorithmic_bytes(sc):
     return {"verlet": verlet, "deform": deform, "refit": refit, "springs": E}
 
 
+def step_kernel_figures(live, tk, ab, rebuild):
+    """Kernel times of the report and the step kernel's achieved bandwidth.
+    live: event timing over the timed steps (cc_sim_get_timing); tk: the
+    same kernel groups one at a time (cc_sim_time_kernels).  The step kernel's
+    live time is used when the timed steps recorded it, else its isolated
+    time; achieved is None when neither is positive."""
+    upd = "build_us" if rebuild else "refit_us"
+    kern = {"verlet": live["verlet_us"], "body_update": live[upd]}
+    iso = {"verlet": tk["verlet_us"], "build" if rebuild else "refit": tk[upd], "deform": tk["body_us"]}
+    if kern["verlet"] > 0:
+        t, dom = kern["verlet"], "live"
+    elif iso["verlet"] > 0:
+        t, dom = iso["verlet"], "isolated"
+    else:
+        return kern, iso, "none", None
+    return kern, iso, dom, ab["verlet"] / (t * 1e-6) / 1e9
+
+
 def read_traffic(path):
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a
@@ -257,6 +275,7 @@ def main():
               "single-GPU kernel; multi-GPU sharding is a design note, not a run)", file=sys.stderr)
         sys.exit(2)
 
+    from simulator_amd import _lib as L
     from simulator_amd import scene as S
     from simulator_amd.simulator import Simulator
 
@@ -264,8 +283,8 @@ def main():
           "reference": lambda: S.load_mesh_scene(os.path.join(ROOT, "tests", "golden", "ref_tshirt_female.npz"))
           }[args.workload]()
     sim = Simulator.from_scene(sc)
-    # HIP events around every step graph and update-lane graph, on the
-    # streams they run on, read while the following blocks run
+    # HIP events around every step graph, single step kernel and body
+    # update, on the streams they run on, read while later work runs
     sim.set_timing(True)
     sim.simulate(args.warmup)
     sim.synchronize()
@@ -277,6 +296,15 @@ def main():
     ms = 1e3 * el / args.steps
     value = sc.n * args.steps / el
     live = sim.timing()
+
+    # the reference's calling pattern: one simulate() call per frame
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.simulate(1)
+    sim.synchronize()
+    el_pf = time.perf_counter() - t0
+    live_pf = sim.timing()
     sim.set_timing(False)
 
     # the same kernel groups one at a time, nothing overlapped
@@ -290,13 +318,12 @@ def main():
     visits, queries = sim.counters()
     sim.set_counting(False)
     sim.check_finite()
+    block = sim.block_steps
+    sim.close()
 
     ab = algorithmic_bytes(sc)
-    kern = {"verlet": live["verlet_us"], "body_update": live["refit_us"]}
-    iso = {"verlet": tk["verlet_us"], "refit": tk["refit_us"], "deform": tk["body_us"]}
-    # the step chain is the critical path: the body updates run beside it
-    dom = "verlet"
-    achieved = ab[dom] / (kern[dom] * 1e-6) / 1e9
+    rebuild = sc.body.mode == L.CC_BODY_REBUILD
+    kern, iso, dom, achieved = step_kernel_figures(live, tk, ab, rebuild)
     step_bytes = ab["verlet"] + ab["refit"] + ab["deform"]
     out = {
         "metric": "vertex-steps/s @262k-vert cloth FP32, % HBM peak; batched & adjoint vs plain",
@@ -315,11 +342,18 @@ def main():
         "config": {"workload": WORKLOADS[args.workload],
                    "vertices": sc.n, "springs_directed": ab["springs"],
                    "body_triangles": int(sc.body.tris.shape[0])},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_verlet", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+                     "traffic": None, "timing": dom},
         "kernels_us": kern,
         "kernels_timed_steps": live["steps"],
         "kernels_isolated_us": iso,
+        "path": "%d graph blocks of %d steps + %d single steps" % (args.steps // block, block, args.steps % block),
+        "per_frame_calls": {"calls": args.steps, "ms_per_step": 1e3 * el_pf / args.steps,
+                            "value": sc.n * args.steps / el_pf,
+                            "verlet_us": live_pf["verlet_us"],
+                            "body_update_us": live_pf["build_us" if rebuild else "refit_us"],
+                            "vs_headline": (sc.n * args.steps / el_pf) / value},
         "step_algorithmic_bytes": step_bytes,
         "step_hbm_frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "bvh_nodes_visited_per_s": visits / el_c,
@@ -328,7 +362,6 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "k_verlet_%s_traffic.txt" % args.workload))
     if traffic:
         out["roofline"]["traffic"] = traffic
-    sim.close()
     if not args.no_batch:
         out["batched"] = batched_leg()
     if not args.no_adjoint:

@@ -236,10 +236,14 @@ int cc_sim_destroy(cc_sim* sim);
    the fused Verlet + spring + collision kernel (verlet.cu:129-161) against
    the body pose of that frame, then the ping-pong swap (:243-251).  A
    REFIT / REBUILD body's poses and LBVHs are computed on the library's own
-   streams, up to one block (cc_sim_block_steps, 24 by default) ahead of the
-   cloth.  Asynchronous on stream (NULL: the simulator's own stream); whole
-   blocks are replayed from captured hipGraphs, the remaining steps are
-   launched one by one. */
+   streams ("update lanes") ahead of the cloth: whole blocks of
+   cc_sim_block_steps steps (24 by default) are replayed from captured
+   hipGraphs while the lanes compute the next block's poses; the remaining
+   steps -- and every call with nsteps = 1, the reference's one call per
+   frame -- are launched one by one, each issuing the pose of a frame a few
+   frames ahead (two per lane) on a lane, so the body update of later frames
+   runs beside this frame's step.  Asynchronous on stream (NULL: the
+   simulator's own stream). */
 int cc_sim_step(cc_sim* sim, int nsteps, void* stream);
 int cc_sim_synchronize(cc_sim* sim);
 /* Steps per captured block (even).  No counterpart in the reference, whose
@@ -348,7 +352,11 @@ int cc_sim_get_normals(cc_sim* sim, float* normals3);
 /* Per-kernel timing of nsteps steps with HIP events on the simulator
    stream, one kernel group at a time (each bracketed, nothing overlapped),
    averaged per launch in microseconds; step_us is then the per-step time of
-   nsteps further steps through the normal (overlapped) path. */
+   nsteps further steps through the normal (overlapped) path.  The
+   simulation advances: the call steps the cloth 2 * nsteps frames forward
+   (nsteps timed one group at a time, then nsteps through cc_sim_step).
+   body_us is the deformation; refit_us (REFIT body) or build_us (REBUILD
+   body) the tree update after it. */
 typedef struct cc_timing {
   double step_us;      /* whole step (graph replay), per step           */
   double verlet_us;    /* fused Verlet + spring + collision kernel      */
@@ -360,10 +368,10 @@ typedef struct cc_timing {
 int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
 
 /* Timing inside the normal path: while enabled, cc_sim_step records HIP
-   events around each graph of fused step kernels and around each body update
-   lane's launches, on the streams they run on, and accumulates their
-   durations (two event sets alternate, so the host reads one block's events
-   while the next block runs).
+   events around each graph of fused step kernels, around each single fused
+   step kernel, and around each body update lane's launches, on the streams
+   they run on, and accumulates their durations (the host reads the events
+   of earlier blocks and steps while later ones run).
    cc_sim_get_timing returns the averages since enabling: verlet_us per
    fused step kernel, refit_us (refitted body) or build_us (rebuilt body)
    per body update including its deformation, steps = steps timed.

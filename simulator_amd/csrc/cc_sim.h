@@ -82,12 +82,32 @@ struct Sim {
   int tset = 0;
   double t_verlet_ms = 0, t_update_ms = 0;
   uint64_t t_steps = 0, t_updates = 0;
+  // Single steps (the reference's one call per frame): the pose of frame g
+  // is computed on lane g % nlanes up to `lookahead` frames ahead of the
+  // step, and ev_pose[slot of g] marks it done; pose_pending says that the
+  // step of the frame in that slot must wait for it.  block_inflight: the
+  // lanes of a step graph block have not been joined yet.
+  int lookahead = 0;
+  hipEvent_t ev_pose[MAX_SLOTS] = {};
+  bool pose_pending[MAX_SLOTS] = {};
+  bool block_inflight = false;
+  // Timing of single steps and of their updates: a ring of event pairs on
+  // the streams they bracket, read once the ring is full or on collection.
+  static constexpr int TRING = 256;
+  hipEvent_t rev[TRING][2] = {};
+  uint8_t rkind[TRING] = {};     // 0: fused step, 1: body update
+  uint64_t r_head = 0, r_tail = 0;
 
   ~Sim() {
     drop_graphs();
     for (auto& set : tev)
       for (hipEvent_t e : set)
         if (e) (void)hipEventDestroy(e);
+    for (auto& pr : rev)
+      for (hipEvent_t e : pr)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_pose)
+      if (e) (void)hipEventDestroy(e);
     if (ev_go) (void)hipEventDestroy(ev_go);
     for (int l = 0; l < MAX_LANES; ++l) {
       if (ev_done[l]) (void)hipEventDestroy(ev_done[l]);
@@ -118,9 +138,17 @@ struct Sim {
     body.update(body.slot_of(g), 0, g, s);
   }
 
+  // Forget the lane work in flight (the device has been synchronised, so
+  // every event has completed): the next step starts the pipeline afresh.
+  void reset_pipeline() {
+    for (bool& p : pose_pending) p = false;
+    lanes_inflight = block_inflight = false;
+  }
+
   // Moving body: drop the poses computed ahead and recompute the current
   // frame's (after a change of frame, parameters or vertices).  Synchronous.
   void refresh_body() {
+    reset_pipeline();
     if (body.moving()) {
       update_direct(frame, stream);
       ready = frame + 1;
@@ -188,22 +216,94 @@ struct Sim {
     tpending[set] = false;
   }
 
-  // the step stream waits for the update lanes of the last block
+  // the step stream waits for all lane work issued so far
   void join_lanes(hipStream_t s) {
     if (!lanes_inflight) return;
     for (int l = 0; l < body.nlanes; ++l) CC_HIP(hipStreamWaitEvent(s, ev_done[l], 0));
-    lanes_inflight = false;
+    lanes_inflight = block_inflight = false;
+    for (bool& p : pose_pending) p = false;
   }
 
+  // ---- timing ring of single steps
+  void ring_collect_one() {
+    const int e = (int)(r_tail % TRING);
+    CC_HIP(hipEventSynchronize(rev[e][1]));
+    float ms = 0;
+    CC_HIP(hipEventElapsedTime(&ms, rev[e][0], rev[e][1]));
+    if (rkind[e] == 0) {
+      t_verlet_ms += ms;
+      ++t_steps;
+    } else {
+      t_update_ms += ms;
+      ++t_updates;
+    }
+    ++r_tail;
+  }
+  void ring_collect_all() {
+    while (r_tail < r_head) ring_collect_one();
+  }
+  // an event pair of the ring (the older pair in its place is read first)
+  hipEvent_t* ring_next(uint8_t kind) {
+    if (r_head - r_tail == (uint64_t)TRING) ring_collect_one();
+    const int e = (int)(r_head % TRING);
+    rkind[e] = kind;
+    ++r_head;
+    return rev[e];
+  }
+
+  // Lane work of the single-step path: the poses of frames [ready, upto]
+  // on lanes g % nlanes.  Each waits for "go", recorded on the step stream
+  // before the current step: the slot of frame g was last read by the step
+  // of frame g - nslots, earlier than the current one (upto < frame + nslots).
+  void issue_ahead(uint64_t upto) {
+    bool waited[MAX_LANES] = {};
+    for (; ready <= upto; ++ready) {
+      const uint64_t g = ready;
+      const int l = (int)(g % (uint64_t)body.nlanes);
+      const int k = body.slot_of(g);
+      hipStream_t ls = lane_s[l];
+      if (!waited[l]) {
+        CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
+        waited[l] = true;
+      }
+      hipEvent_t* te = timing ? ring_next(1) : nullptr;
+      if (te) CC_HIP(hipEventRecord(te[0], ls));
+      body.update(k, l, g, ls);
+      if (te) CC_HIP(hipEventRecord(te[1], ls));
+      CC_HIP(hipEventRecord(ev_pose[k], ls));
+      CC_HIP(hipEventRecord(ev_done[l], ls));
+      pose_pending[k] = true;
+      lanes_inflight = true;
+    }
+  }
+
+  // One frame (the reference's simulate(), simulation.cu:141-150).  A
+  // moving body's pose of this frame was issued on a lane by earlier calls
+  // (computed here on the step stream only when nothing is ahead); the
+  // poses of the next `lookahead` frames are issued on the lanes, where they
+  // run beside this step and the following ones.
   void step_single(hipStream_t s) {
-    join_lanes(s);
-    launch_verlet(body.slot_of(frame), s);
+    const int k = body.slot_of(frame);
+    const bool pipe = body.moving();
+    if (pipe) {
+      if (block_inflight) join_lanes(s);
+      if (ready <= frame) {
+        update_direct(frame, s);
+        ready = frame + 1;
+      }
+      if (pose_pending[k]) {
+        CC_HIP(hipStreamWaitEvent(s, ev_pose[k], 0));
+        pose_pending[k] = false;
+      }
+      CC_HIP(hipEventRecord(ev_go, s));
+    }
+    hipEvent_t* te = timing ? ring_next(0) : nullptr;
+    if (te) CC_HIP(hipEventRecord(te[0], s));
+    launch_verlet(k, s);
+    if (te) CC_HIP(hipEventRecord(te[1], s));
+    if (pipe) issue_ahead(frame + (uint64_t)lookahead);
     cur ^= 1;
     ++frame;
-    if (body.moving() && ready <= frame) {
-      update_direct(frame, s);
-      ready = frame + 1;
-    }
   }
 
   // One block of graph_steps steps.  A moving body's poses for these frames
@@ -225,8 +325,13 @@ struct Sim {
     if (timing) collect(tset);
     const bool pipe = body.moving();
     if (pipe) {
+      // poses of this block not yet issued: on the lanes, in parallel (their
+      // slots were last read by steps before this one), then joined
+      if (ready < frame + G) {
+        CC_HIP(hipEventRecord(ev_go, s));
+        issue_ahead(frame + G - 1);
+      }
       join_lanes(s);
-      for (; ready < frame + G; ++ready) update_direct(ready, s);
       CC_HIP(hipEventRecord(ev_go, s));
     }
     // The step graph is submitted first: submitting a graph to a stream that
@@ -251,7 +356,7 @@ struct Sim {
         if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
         CC_HIP(hipEventRecord(ev_done[l], ls));
       }
-      lanes_inflight = true;
+      lanes_inflight = block_inflight = true;
       ready = frame + 2 * G;
     }
     if (te) {

@@ -384,6 +384,12 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
           CC_HIP(hipEventCreate(&S.ev_done[l]));
         }
         CC_HIP(hipEventCreate(&S.ev_go));
+        for (int k = 0; k < S.body.nslots; ++k) CC_HIP(hipEventCreate(&S.ev_pose[k]));
+        // single steps keep two updates per lane in flight
+        S.lookahead = 2 * S.body.nlanes;
+        if (const char* e = std::getenv("CC_LOOKAHEAD")) S.lookahead = std::atoi(e);
+        CC_REQUIRE(S.lookahead >= 1 && S.lookahead < S.body.nslots,
+                   "cc_sim_create: CC_LOOKAHEAD must be 1..slots-1");
       }
     }
     CC_HIP(hipStreamSynchronize(s));
@@ -817,6 +823,7 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     double t_body = 0, t_upd = 0, t_ver = 0;
     // the work of a step one kernel group at a time: the body update of the
     // next frame, then the step of this frame
+    S.reset_pipeline();
     S.ready = S.frame + 1;
     for (int it = 0; it < nsteps; ++it) {
       const int nk = S.body.slot_of(S.frame + 1);
@@ -863,13 +870,18 @@ int cc_sim_set_timing(cc_sim* sim, int enable) {
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
     for (int e = 0; e < 2; ++e) S.collect(e);
+    S.ring_collect_all();
     S.timing = enable != 0;
     if (S.timing && !S.tev[0][0])
       for (auto& set : S.tev)
         for (hipEvent_t& ev : set) CC_HIP(hipEventCreate(&ev));
+    if (S.timing && !S.rev[0][0])
+      for (auto& pr : S.rev)
+        for (hipEvent_t& ev : pr) CC_HIP(hipEventCreate(&ev));
     S.t_verlet_ms = S.t_update_ms = 0;
     S.t_steps = S.t_updates = 0;
     S.tpending[0] = S.tpending[1] = false;
+    S.r_head = S.r_tail = 0;
   });
 }
 
@@ -878,6 +890,7 @@ int cc_sim_get_timing(cc_sim* sim, cc_timing* out) {
     CC_REQUIRE(sim && out, "cc_sim_get_timing: null");
     Sim& S = sim->s;
     for (int e = 0; e < 2; ++e) S.collect(e);
+    S.ring_collect_all();
     std::memset(out, 0, sizeof(*out));
     out->steps = (uint32_t)S.t_steps;
     if (S.t_steps) out->verlet_us = 1e3 * S.t_verlet_ms / (double)S.t_steps;

@@ -171,7 +171,7 @@ def test_multi_step_graph_equals_single_steps(gpu, mode):
     b.close()
 
 
-def test_in_graph_timing_reports_every_graph_step(gpu):
+def test_in_graph_timing_reports_every_step(gpu):
     sc = moving_scene(L.CC_BODY_REFIT, side=32)
     sim = Simulator.from_scene(sc)
     G = sim.block_steps
@@ -179,7 +179,7 @@ def test_in_graph_timing_reports_every_graph_step(gpu):
     sim.simulate(5 * G + 3)           # five graphs, then three single steps
     t = sim.timing()
     say(f"[body] in-graph timing: {t}")
-    assert t["steps"] == 5 * G
+    assert t["steps"] == 5 * G + 3
     assert t["verlet_us"] > 0 and t["refit_us"] > 0
     x = sim.get_state()[0]
     sim.set_timing(False)
@@ -188,3 +188,23 @@ def test_in_graph_timing_reports_every_graph_step(gpu):
     np.testing.assert_array_equal(ref.get_state()[0], x)     # timing changes no result
     sim.close()
     ref.close()
+
+
+@pytest.mark.parametrize("mode", [L.CC_BODY_REFIT, L.CC_BODY_REBUILD])
+def test_single_step_calls_are_timed(gpu, mode):
+    """One call per frame (the reference's calling pattern): every step
+    kernel and every body update issued on the lanes is timed."""
+    sc = moving_scene(mode, side=32)
+    sim = Simulator.from_scene(sc)
+    sim.simulate(3)
+    sim.synchronize()
+    sim.set_timing(True)
+    for _ in range(20):
+        sim.simulate(1)
+    t = sim.timing()
+    say(f"[body] single-step timing (mode {mode}): {t}")
+    assert t["steps"] == 20
+    upd = t["refit_us"] if mode == L.CC_BODY_REFIT else t["build_us"]
+    assert t["verlet_us"] > 0 and upd > 0
+    sim.set_timing(False)
+    sim.close()
