@@ -31,6 +31,9 @@ extern "C" {
 #define CC_ERR_ALLOC 3       /* host or device allocation failed           */
 #define CC_ERR_NONFINITE 4   /* NaN / Inf detected in the state            */
 #define CC_ERR_UNSUPPORTED 5 /* option not available in this build         */
+#define CC_ERR_TRAVERSAL 6   /* a collision query overflowed its traversal
+                                stack (taken as a miss) or hit its visit
+                                bound (corrupted tree)                      */
 
 /* spring record, s_spring in src/spring.h:8-12 (8 bytes) */
 typedef struct cc_spring {
@@ -278,8 +281,16 @@ int cc_sim_get_debug(cc_sim* sim, float* pre3, int32_t* hit, uint8_t* inside);
 int cc_sim_set_counting(cc_sim* sim, int enable);
 int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries);
 
-/* Finite check of x_cur: CC_ERR_NONFINITE if any NaN / Inf. */
+/* Finite check of x_cur: CC_ERR_NONFINITE if any NaN / Inf; before that,
+   CC_ERR_TRAVERSAL if a step's collision query overflowed its traversal
+   stack since the last check (the reference's D_BVH::intersect,
+   cudaBRT.cu:243-245, has no overflow test; here the query is taken as a
+   miss and recorded, and this call reports and clears the record). */
 int cc_sim_check_finite(cc_sim* sim);
+/* Test hook (no counterpart in the reference): lower the traversal stack
+   limit (1..64, default 64, the most a walk over a valid tree defers) so
+   that the overflow record and its report can be exercised. */
+int cc_sim_set_stack_limit(cc_sim* sim, int limit);
 
 /* ------------------------------------------------------------------ body */
 
@@ -424,6 +435,8 @@ int cc_batch_destroy(cc_batch* b);
    own stream). */
 int cc_batch_step(cc_batch* b, int nsteps, void* stream);
 int cc_batch_synchronize(cc_batch* b);
+/* cc_sim_check_finite over every instance of the batch. */
+int cc_batch_check(cc_batch* b);
 /* cc_sim_set_pins / cc_sim_get_state for instance inst. */
 int cc_batch_set_pins(cc_batch* b, uint32_t inst, const uint32_t* idx, uint32_t k);
 int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, float* contact_n);

@@ -434,18 +434,22 @@ def rollout_f64(x0, v0, springs, prm64, pins=None, bvh=None, nsteps=1, trajector
     return (xf, tp, tq, tc, to) if trajectory else xf
 
 
-def rollout_f64_frozen(x0, v0, springs, prm64, pins, traj_c, traj_a):
-    """FP64 rollout with the recorded contacts held fixed; returns x_final."""
+def rollout_f64_frozen(x0, v0, springs, prm64, pins, traj_c, traj_a, trajectory=False):
+    """FP64 rollout with the recorded contacts held fixed; returns x_final,
+    or (x_final, traj_p, traj_q)."""
     n = x0.shape[0]
     nsteps = traj_c.shape[0] - 1
     x0 = _c(x0, np.float64)
     v0 = _c(np.zeros_like(x0) if v0 is None else v0, np.float64)
     xf = np.zeros((n, 3), np.float64)
+    tp = tq = None
+    if trajectory:
+        tp, tq = (np.zeros((nsteps + 1, n, 3), np.float64) for _ in range(2))
     sp = [(_c(r, np.uint32), _c(c, SPRING_DTYPE)) for r, c in springs]
     pins_a = None if pins is None else _c(pins, np.uint8)
     olib().or_rollout_f64_frozen(n, _p(x0), _p(v0), *_springs_args(sp), C.byref(prm64), _p(pins_a), nsteps,
-                                 _p(_c(traj_c, np.float64)), _p(_c(traj_a, np.float64)), _p(xf))
-    return xf
+                                 _p(_c(traj_c, np.float64)), _p(_c(traj_a, np.float64)), _p(xf), _p(tp), _p(tq))
+    return (xf, tp, tq) if trajectory else xf
 
 
 def adjoint_f64(springs, prm64, pins, traj_p, traj_q, traj_c, dl_dxf):

@@ -1067,7 +1067,8 @@ void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
                            const uint32_t* row1, const or_spring* col1,
                            const uint32_t* row2, const or_spring* col2,
                            const or_params64* P, const uint8_t* pins, uint32_t nsteps,
-                           const double* traj_c, const double* traj_a, double* x_final) {
+                           const double* traj_c, const double* traj_a, double* x_final,
+                           double* traj_p, double* traj_q) {
   const uint32_t* rows[3] = {row0, row1, row2};
   const or_spring* cols[3] = {col0, col1, col2};
   const size_t N = 3 * (size_t)n;
@@ -1078,6 +1079,8 @@ void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
     q[k] = x0[k] - (v0 ? v0[k] : 0.0) * P->dt;
   }
   for (uint32_t t = 0; t < nsteps; ++t) {
+    if (traj_p) memcpy(traj_p + t * N, p, N * sizeof(double));
+    if (traj_q) memcpy(traj_q + t * N, q, N * sizeof(double));
     const double* c = traj_c + t * N;
     const double* cn = traj_c + (t + 1) * N;
     const double* an = traj_a + (t + 1) * N;
@@ -1103,6 +1106,8 @@ void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
     tmp = p; p = np; np = tmp;
     tmp = q; q = nq; nq = tmp;
   }
+  if (traj_p) memcpy(traj_p + (size_t)nsteps * N, p, N * sizeof(double));
+  if (traj_q) memcpy(traj_q + (size_t)nsteps * N, q, N * sizeof(double));
   memcpy(x_final, p, N * sizeof(double));
   free(buf);
 }

@@ -215,13 +215,15 @@ void or_rollout_f64(uint32_t n, const double* x0, const double* v0,
    traj_a[t+1] with normal n in step t whatever its position, the others
    never are; the force projection uses traj_c[t].  The function whose
    derivative or_adjoint_f64 computes; at the recorded parameters it equals
-   the rollout bit for bit. */
+   the rollout bit for bit.  traj_p / traj_q (may be NULL): its positions
+   and previous positions, (nsteps + 1) x n x 3. */
 void or_rollout_f64_frozen(uint32_t n, const double* x0, const double* v0,
                            const uint32_t* row0, const or_spring* col0,
                            const uint32_t* row1, const or_spring* col1,
                            const uint32_t* row2, const or_spring* col2,
                            const or_params64* prm, const uint8_t* pins, uint32_t nsteps,
-                           const double* traj_c, const double* traj_a, double* x_final);
+                           const double* traj_c, const double* traj_a, double* x_final,
+                           double* traj_p, double* traj_q);
 
 /* Adjoint of or_rollout_f64 along a recorded trajectory, with the contacts
    of the forward held fixed: a vertex projected in a step is moved by the

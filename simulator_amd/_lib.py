@@ -17,6 +17,7 @@ CC_ERR_HIP = 2
 CC_ERR_ALLOC = 3
 CC_ERR_NONFINITE = 4
 CC_ERR_UNSUPPORTED = 5
+CC_ERR_TRAVERSAL = 6
 
 CC_BODY_NONE = 0
 CC_BODY_STATIC = 1
@@ -119,6 +120,7 @@ PROTOTYPES = {
     "cc_sim_set_counting": (C.c_int, [P, C.c_int]),
     "cc_sim_get_counters": (C.c_int, [P, C.POINTER(u64), C.POINTER(u64)]),
     "cc_sim_check_finite": (C.c_int, [P]),
+    "cc_sim_set_stack_limit": (C.c_int, [P, C.c_int]),
     "cc_body_set_vertices": (C.c_int, [P, P]),
     "cc_body_get_vertices": (C.c_int, [P, P]),
     "cc_body_get_motion": (C.c_int, [P, P, P, P]),
@@ -150,6 +152,7 @@ PROTOTYPES = {
     "cc_batch_destroy": (C.c_int, [P]),
     "cc_batch_step": (C.c_int, [P, C.c_int, P]),
     "cc_batch_synchronize": (C.c_int, [P]),
+    "cc_batch_check": (C.c_int, [P]),
     "cc_batch_set_pins": (C.c_int, [P, C.c_uint32, P, C.c_uint32]),
     "cc_batch_get_state": (C.c_int, [P, C.c_uint32, P, P, P]),
     "cc_batch_info": (C.c_int, [P, P, P, P]),

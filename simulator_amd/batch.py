@@ -118,6 +118,10 @@ class Batch:
     def simulate(self, nsteps=1, stream=None):
         L.check(L.lib().cc_batch_step(self._h, int(nsteps), stream))
 
+    def check(self):
+        """cc_sim_check_finite over every instance (cc_batch_check)."""
+        L.check(L.lib().cc_batch_check(self._h))
+
     def synchronize(self):
         L.check(L.lib().cc_batch_synchronize(self._h))
 

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(TILE) void k_verlet_batch(const float* __restrict__
   const InstDesc d = insts[b.x];
   const uint32_t i = b.y + threadIdx.x;
   uint32_t visits = 0;
-  step_vertex<false, EW, false>(i, sin_, sout, contact, contact, row, rec, d.a, d.bvh, Spill{spill + i, ntotal}, visits, nullptr,
+  step_vertex<false, EW, false>(i, sin_, sout, contact, contact, row, rec, d.a, d.bvh, spill_of(spill, i, ntotal), visits, nullptr,
                      nullptr, nullptr);
 }
 
@@ -51,6 +51,7 @@ struct Batch {
   DBuf<uint2> ell;                           // [ELL_W * ntotal] the records, fixed width (when ell_w)
   int ell_w = 0;
   DBuf<int32_t> spill;
+  DBuf<uint32_t> flag;                       // [1] finite-check scratch
   BatchBodies bodies;
   DBuf<InstDesc> insts[2];                   // per body slot (frame g uses slot g % 2 == cur)
   uint64_t frame = 0;
@@ -306,7 +307,7 @@ int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, con
       B.ell.upload(er.data(), er.size(), s);
       B.ell_w = ELL_W;
     }
-    B.spill.alloc((size_t)STACK_SPILL * B.ntotal);
+    alloc_spill(B.spill, B.ntotal);
     // bodies: concatenated, frame 0 built for all of them
     B.bodies.init(bodies, params, count, s);
     std::vector<InstDesc> desc(count);
@@ -364,6 +365,16 @@ int cc_batch_synchronize(cc_batch* b) {
     CC_REQUIRE(b, "cc_batch_synchronize: null");
     CC_HIP(hipStreamSynchronize(b->b.stream));
     CC_HIP(hipDeviceSynchronize());
+  });
+}
+
+int cc_batch_check(cc_batch* b) {
+  return guard([&] {
+    CC_REQUIRE(b, "cc_batch_check: null");
+    Batch& B = b->b;
+    CC_HIP(hipDeviceSynchronize());
+    if (!B.flag.p) B.flag.alloc(1);
+    check_state(B.state[B.cur].p, B.ntotal, B.spill, B.ntotal, B.flag.p, B.stream, "cc_batch_check");
   });
 }
 

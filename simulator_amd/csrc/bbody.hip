@@ -303,7 +303,7 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __r
   const InstDesc dd = st.insts[b.x];
   const uint32_t i = b.y + lt;
   uint32_t visits = 0;
-  step_vertex<false, EW, false, 6>(i, st.in, st.out, st.contact, st.contact, st.row, st.rec, dd.a, dd.bvh, Spill{st.spill + i, st.ntotal},
+  step_vertex<false, EW, false, 6>(i, st.in, st.out, st.contact, st.contact, st.row, st.rec, dd.a, dd.bvh, spill_of(st.spill, i, st.ntotal),
                      visits, nullptr, nullptr, nullptr);
 }
 

@@ -24,20 +24,55 @@ constexpr int WSTK_LDS = 16;
 constexpr int WSTK_LANES = 256;
 // the wide walk defers at most 64 entries: the ones past the LDS part fit
 static_assert(64 - WSTK_LDS <= STACK_SPILL, "wide-walk stack beyond LDS exceeds the spill area");
+// A spill area of L lanes is STACK_SPILL * L entries followed by a tail of
+// SPILL_TAIL words: [0] the traversal faults of the launches that used it
+// (TRAV_OVERFLOW: a walk needed more deferred entries than the stack limit,
+// its query was then taken as a miss; TRAV_BOUND: a walk reached its visit
+// bound, which only a corrupted tree can), [1] the stack limit, 64 (the
+// most a walk of a valid tree defers) unless a test lowers it to force an
+// overflow.  The host reads and clears the faults (cc_sim_check_finite).
+constexpr int SPILL_TAIL = 2;
+constexpr uint32_t TRAV_OVERFLOW = 1u, TRAV_BOUND = 2u;
+constexpr int STACK_LIMIT = 64;
 struct Spill {
   int32_t* p;
   uint32_t stride;
   int32_t* lds = nullptr;   // this lane's entry 0 (kernels that walk the four-wide form)
+  int32_t* tail = nullptr;  // the area's tail (base + STACK_SPILL * stride)
 };
+// the spill area of lane i of a launch over `stride` lanes
+__device__ inline Spill spill_of(int32_t* base, uint32_t i, uint32_t stride, int32_t* lds = nullptr) {
+  return Spill{base + i, stride, lds, base + (size_t)STACK_SPILL * stride};
+}
+__device__ inline void trav_fault(const Spill& s, uint32_t bit) {
+  atomicOr(reinterpret_cast<uint32_t*>(s.tail), bit);
+}
+__device__ inline int stack_limit(const Spill& s) { return min(s.tail[1], STACK_LIMIT); }
+
+// host: a spill area for `lanes` lanes, its tail set to no faults and the
+// default stack limit (set-up only: a synchronous copy)
+inline void alloc_spill(DBuf<int32_t>& b, size_t lanes) {
+  b.alloc((size_t)STACK_SPILL * lanes + SPILL_TAIL);
+  const int32_t tail[SPILL_TAIL] = {0, STACK_LIMIT};
+  CC_HIP(hipMemcpy(b.p + (size_t)STACK_SPILL * lanes, tail, sizeof(tail), hipMemcpyHostToDevice));
+}
+
+// host: the finite check of n {x_cur, x_prev} records at `state` and the
+// traversal faults recorded in the spill area of `lanes` lanes (read, then
+// cleared); throws CC_ERR_NONFINITE or CC_ERR_TRAVERSAL.  flag: one device
+// word of scratch.  Synchronous on s.
+void check_state(const float* state, uint32_t n, DBuf<int32_t>& spill, size_t lanes, uint32_t* flag,
+                 hipStream_t s, const char* who);
 
 // Deferred nodes: the top four in registers (static shifts, no indexed
 // private memory), older ones in the spill area.
 struct TravStack {
   int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int sp = 0;
+  int limit = STACK_LIMIT;
   Spill spill;
   __device__ bool push(int32_t v) {
-    if (sp == 64) return false;          // unreachable for depth < 64
+    if (sp >= limit) return false;       // unreachable for a valid tree at the default limit
     if (sp >= 4) spill.p[(size_t)(sp - 4) * spill.stride] = s3;
     s3 = s2;
     s2 = s1;
@@ -83,7 +118,10 @@ __device__ inline int32_t trav_step(f3 p, float4 q0, float4 q1, float4 q2, int4 
   if (hitA | hitB) return leaf;
   const bool tL = oL & (ca >= 0), tR = oR & (cb >= 0);
   if (tL) {
-    if (tR && !st.push(cb)) return STEP_MISS;
+    if (tR && !st.push(cb)) {
+      trav_fault(st.spill, TRAV_OVERFLOW);
+      return STEP_MISS;
+    }
     next = ca;
     return STEP_DESCEND;
   }
@@ -162,7 +200,8 @@ __device__ inline int32_t wstk_get(const Spill& sp, int k) {
 // a SIMD walk at once.
 enum : int { WIDE_NEXT = -1, WIDE_MISS = -2 };
 __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4, float4 q5,
-                                     float4 q6, float4 q7, const Spill& spill, int& sp, int32_t& node) {
+                                     float4 q6, float4 q7, const Spill& spill, int& sp, int32_t& node,
+                                     int limit) {
   // Kept as booleans (lane masks): the first candidate and the deferred set
   // follow from them by selects, with no per-lane bit mask to build.
   const bool i0 = in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y), i1 = in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y),
@@ -184,7 +223,10 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
   const int32_t x0 = r3 ? c3 : (r2 ? c2 : c1);
   const int32_t x1 = r3 ? (r2 ? c2 : c1) : c1;
   const int k = (int)r1 + (int)r2 + (int)r3;
-  if (sp + k > 64) return WIDE_MISS;    // unreachable: the set-up bounds the deferrals by 64
+  if (sp + k > limit) {                 // unreachable at the default limit: the set-up bounds
+    trav_fault(spill, TRAV_OVERFLOW);   // the deferrals by 64
+    return WIDE_MISS;
+  }
   if (sp + 3 <= WSTK_LDS) {
     int32_t* t = spill.lds + sp * WSTK_LANES;
     t[0] = x0;
@@ -203,9 +245,12 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
 __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill, const WideRoot& rt) {
   int sp = 0;
   int32_t node = 0;
+  const int limit = stack_limit(spill);
   ++visits;
-  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node);
-  for (uint32_t guard = 0; r == WIDE_NEXT && guard < b.m; ++guard) {
+  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node,
+                         limit);
+  uint32_t guard = 0;
+  for (; r == WIDE_NEXT && guard < b.m; ++guard) {
     // a 32-bit byte offset (records < 4 GiB): the loads take the base in
     // scalar registers
     const float4* nd = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(b.wide) +
@@ -217,8 +262,9 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
     // memory round trip per descending visit.
     asm volatile("" ::"v"(q1.z), "v"(q3.z), "v"(q5.z), "v"(q7.z));
     ++visits;
-    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, spill, sp, node);
+    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, spill, sp, node, limit);
   }
+  if (r == WIDE_NEXT) trav_fault(spill, TRAV_BOUND);
   return r >= 0 ? r : -1;
 }
 
@@ -238,6 +284,7 @@ __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits
   if (WIDE && b.wide) return wide_first_hit(b, p, visits, spill, rt);
   TravStack st;
   st.spill = spill;
+  st.limit = stack_limit(spill);
   int32_t node = 0;
   // A well-formed tree visits each internal node at most once; the bound
   // only guards against a corrupted tree (the loop always terminates).
@@ -250,6 +297,7 @@ __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits
     if (r >= 0) return r;
     if (r == STEP_MISS) return -1;
   }
+  trav_fault(spill, TRAV_BOUND);
   return -1;
 }
 

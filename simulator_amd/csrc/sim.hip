@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_verlet(const float* __restrict__ sin_, 
       lds = wstk + threadIdx.x;
     }
     step_vertex<DEBUG, EW, WIDE>(i, sin_, sout, contact_in, contact_out, row, rec, a, bvh,
-                                 Spill{spill + i, n, lds}, visits, dbg_pre, dbg_hit, dbg_in);
+                                 spill_of(spill, i, n, lds), visits, dbg_pre, dbg_hit, dbg_in);
   }
   if (COUNT) {
     // block sum into this block's own counter pair (no shared atomics)
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float
   f3 p = ld3(pts + 3 * (size_t)i), nc;
   int32_t h;
   uint32_t visits = 0;
-  const bool in = collide_project(bvh, p, nc, h, visits, Spill{spill + i, n, wstk + threadIdx.x}, load_wide_root(bvh));
+  const bool in = collide_project(bvh, p, nc, h, visits, spill_of(spill, i, n, wstk + threadIdx.x), load_wide_root(bvh));
   hit[i] = h;
   ins[i] = in ? 1 : 0;
   st3(out_pos + 3 * (size_t)i, p);
@@ -112,6 +112,27 @@ __global__ void k_vbo(const float* __restrict__ s, const uint32_t* __restrict__ 
   nn = normalize(nn);
   st3(vbo_nrm + 3 * (size_t)i, nn);
   st3(keep_n + 3 * (size_t)i, nn);
+}
+
+void check_state(const float* state, uint32_t n, DBuf<int32_t>& spill, size_t lanes, uint32_t* flag,
+                 hipStream_t s, const char* who) {
+  CC_HIP(hipMemsetAsync(flag, 0, 4, s));
+  if (n) k_finite<<<grid_for(n, 256), 256, 0, s>>>(state, n, flag);
+  CC_HIP(hipGetLastError());
+  uint32_t bad = 0, fault = 0;
+  int32_t* tail = spill.p + (size_t)STACK_SPILL * lanes;
+  CC_HIP(hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, s));
+  CC_HIP(hipMemcpyAsync(&fault, tail, 4, hipMemcpyDeviceToHost, s));
+  CC_HIP(hipStreamSynchronize(s));
+  if (fault) {
+    CC_HIP(hipMemsetAsync(tail, 0, 4, s));
+    CC_HIP(hipStreamSynchronize(s));
+    throw Error(CC_ERR_TRAVERSAL, std::string(who) +
+                    ((fault & TRAV_OVERFLOW) ? ": traversal stack overflow: a collision query needed more deferred "
+                                               "nodes than the stack holds and was taken as a miss"
+                                             : ": traversal visit bound reached (corrupted tree)"));
+  }
+  if (bad) throw Error(CC_ERR_NONFINITE, std::string(who) + ": non-finite position found");
 }
 
 StepArgs step_args(const cc_params& prm, uint32_t ntypes) {
@@ -327,7 +348,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       S.ell.upload(er.data(), er.size(), s);
       S.ell_w = (int)w;
     }
-    S.spill.alloc((size_t)STACK_SPILL * S.n);
+    alloc_spill(S.spill, S.n);
     S.state[0].upload(packed.data(), packed.size(), s);
     S.state[1].alloc(packed.size());
     S.state[1].zero(s);
@@ -571,19 +592,22 @@ int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries)
 }
 
 int cc_sim_check_finite(cc_sim* sim) {
-  int rc = guard([&] {
+  return guard([&] {
     CC_REQUIRE(sim, "cc_sim_check_finite: null");
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemset(S.flag.p, 0, 4));
-    k_finite<<<grid_for(S.n, 256), 256, 0, S.stream>>>(S.state[S.cur].p, S.n, S.flag.p);
-    CC_HIP(hipGetLastError());
-    uint32_t bad = 0;
-    CC_HIP(hipMemcpyAsync(&bad, S.flag.p, 4, hipMemcpyDeviceToHost, S.stream));
-    CC_HIP(hipStreamSynchronize(S.stream));
-    if (bad) throw Error(CC_ERR_NONFINITE, "cc_sim_check_finite: non-finite position found");
+    check_state(S.state[S.cur].p, S.n, S.spill, S.n, S.flag.p, S.stream, "cc_sim_check_finite");
   });
-  return rc;
+}
+
+int cc_sim_set_stack_limit(cc_sim* sim, int limit) {
+  return guard([&] {
+    CC_REQUIRE(sim && limit >= 1 && limit <= STACK_LIMIT, "cc_sim_set_stack_limit: limit must be 1..64");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    const int32_t v = limit;
+    CC_HIP(hipMemcpy(S.spill.p + (size_t)STACK_SPILL * S.n + 1, &v, 4, hipMemcpyHostToDevice));
+  });
 }
 
 int cc_body_set_vertices(cc_sim* sim, const float* verts3) {
@@ -708,7 +732,7 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     DBuf<uint8_t> in;
     DBuf<int32_t> sp;
     p.upload(pts3, 3 * (size_t)n, s);
-    sp.alloc((size_t)STACK_SPILL * n);
+    alloc_spill(sp, n);
     op.alloc(3 * (size_t)n);
     on.alloc(3 * (size_t)n);
     h.alloc(n);
@@ -719,7 +743,10 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(out_pos, op.p, 12 * (size_t)n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(out_n, on.p, 12 * (size_t)n, hipMemcpyDeviceToHost, s));
+    uint32_t fault = 0;
+    CC_HIP(hipMemcpyAsync(&fault, sp.p + (size_t)STACK_SPILL * n, 4, hipMemcpyDeviceToHost, s));
     CC_HIP(hipStreamSynchronize(s));
+    if (fault) throw Error(CC_ERR_TRAVERSAL, "cc_collide_points: traversal fault (stack overflow or visit bound)");
   });
 }
 

@@ -304,7 +304,13 @@ class Simulator:
         return a.value, b.value
 
     def check_finite(self):
+        """Raise on a non-finite position (CC_ERR_NONFINITE) or a traversal
+        fault since the last check (CC_ERR_TRAVERSAL)."""
         L.check(L.lib().cc_sim_check_finite(self._h))
+
+    def set_stack_limit(self, limit):
+        """Test hook: the traversal stack limit (1..64, default 64)."""
+        L.check(L.lib().cc_sim_set_stack_limit(self._h, int(limit)))
 
     # ---- body / LBVH
     @property

@@ -31,3 +31,12 @@ def gpu():
     if not gpu_available():
         pytest.skip("no GPU")
     return True
+
+
+@pytest.fixture(params=["host", pytest.param("gpu box", marks=pytest.mark.gpu)])
+def both_suites(request):
+    """A module that uses this fixture runs each of its tests twice: in the
+    CPU suite and again in the GPU suite, so that the parity of host-side
+    rows (spring builders, OBJ reader, boundary springs) is part of the GPU
+    box's run too."""
+    return request.param

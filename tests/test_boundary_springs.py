@@ -17,6 +17,9 @@ from simulator_amd import _lib as L
 from simulator_amd import scene as S
 from simulator_amd.simulator import ObjLoader
 
+# every test here runs in the CPU suite and again in the GPU suite
+pytestmark = pytest.mark.usefixtures("both_suites")
+
 
 def test_kd_nearest_is_the_nearest_without_ties():
     rng = np.random.default_rng(4)

@@ -126,3 +126,93 @@ def test_torch_function_gives_the_adjoint(c4):
     np.testing.assert_array_equal(v0.grad.numpy(), dv0)
     assert theta.grad.dtype == torch.float64 and np.all(np.isfinite(dth)) and np.any(dth != 0)
     adj.close()
+
+
+def test_adjoint_with_active_contacts(gpu):
+    """A rollout that falls onto the sphere (32x32 cloth starting 1 cm above
+    its top, 120 steps): the contact-projection adjoint runs.  The GPU
+    adjoint against (1) the FP64 adjoint along the GPU's own trajectory and
+    contact record, and (2) FP64 central differences of the rollout with the
+    GPU's contacts held fixed as planes (normal and anchor per step and
+    vertex) -- the function the adjoint differentiates."""
+    nsteps = 120
+    sc = S.small(side=32, y0=0.81, steps=nsteps)
+    pins = np.zeros(sc.n, np.uint8)
+    pins[sc.pins] = 1
+    # the GPU trajectory with its contact record, one step at a time
+    sim = Simulator.from_scene(sc)
+    body = sim.body_vertices()
+    d = oracle_bvh(body, sc.body.tris, sc.params.box_depth)
+    sim.set_debug(True)
+    tp = np.zeros((nsteps + 1, sc.n, 3))
+    tq, tc, ta = np.zeros_like(tp), np.zeros_like(tp), np.zeros_like(tp)
+    x, xp, c = sim.get_state()
+    tp[0], tq[0], tc[0] = x, xp, c
+    inside = 0
+    for t in range(nsteps):
+        sim.simulate(1)
+        _, hit, ins = sim.get_debug()
+        x, xp, c = sim.get_state()
+        tp[t + 1], tq[t + 1], tc[t + 1] = x, xp, c
+        on = ins == 1
+        tri = sc.body.tris[d["order"][hit[on]]]
+        ta[t + 1][on] = body[tri[:, 0]]                  # the plane's anchor: the hit triangle's v0
+        inside += int(on.sum())
+    sim.close()
+    say(f"[adjoint] contacts: {inside} vertex-steps in contact over {nsteps} steps")
+    assert inside > 500
+    # the adjoint's forward is the same rollout
+    sim2 = Simulator.from_scene(sc)
+    adj = Adjoint(sim2, nsteps, 20)
+    xf = adj.forward(sc.pos, None, THETA0)
+    np.testing.assert_array_equal(xf, tp[nsteps].astype(np.float32))
+    w = np.random.default_rng(7).standard_normal((sc.n, 3)).astype(np.float32)
+    dth, dv0, _ = adj.backward(w)
+    prm = O.params64_from(sc.params, k=THETA0[:3], damp=THETA0[3], rest_scale=THETA0[4])
+    dth64, dv064, _ = O.adjoint_f64(sc.springs, prm, pins, tp, tq, tc, w.astype(np.float64))
+    rel = np.abs(dth - dth64) / np.abs(dth64)
+    for k in range(5):
+        say(f"[adjoint]   contacts: dL/d{NAMES[k]:10s} gpu {dth[k]: .9e}  fp64 adjoint {dth64[k]: .9e}  "
+            f"rel {rel[k]:.1e}")
+    rv = np.linalg.norm(dv0 - dv064) / np.linalg.norm(dv064)
+    say(f"[adjoint]   contacts: dL/dv0 relative error vs fp64 adjoint {rv:.1e}")
+    assert np.all(rel <= 1e-3) and rv <= 1e-3
+    # The frozen-contact rollout (the GPU's planes held fixed) in FP64:
+    # central differences of it pin the FP64 adjoint along its own trajectory
+    # to 1e-5, and give the GPU adjoint within 1e-3 of the gradient's norm.
+    # (Per component the GPU adjoint differs from these differences by how
+    # much the derivative moves between the FP32 and the FP64 trajectories --
+    # up to ~1e-2 for the structural stiffness, whose gradient is a small sum
+    # of near-cancelling stretches; the FP64 adjoint evaluated along the GPU's
+    # trajectory shows the same difference, printed.)
+    def frozen(theta, v0=None, trajectory=False):
+        p = O.params64_from(sc.params, k=theta[:3], damp=theta[3], rest_scale=theta[4])
+        return O.rollout_f64_frozen(sc.pos, v0, sc.springs, p, pins, tc, ta, trajectory=trajectory)
+    xz, fp, fq = frozen(THETA0, trajectory=True)
+    assert np.abs(xz - tp[nsteps]).max() < 1e-4
+    dthz, dv0z, _ = O.adjoint_f64(sc.springs, prm, pins, fp, fq, tc, w.astype(np.float64))
+    fd = np.zeros(5)
+    for k in range(5):
+        # relative step 1e-4: smaller steps drown the stiffness gradients
+        # (1e-6 of the others) in the rollout's FP64 roundoff
+        hk = 1e-4 * abs(THETA0[k])
+        a, b = THETA0.copy(), THETA0.copy()
+        a[k] += hk
+        b[k] -= hk
+        fd[k] = float(np.sum(w * (frozen(a) - frozen(b)))) / (2 * hk)
+        say(f"[adjoint]   contacts: d(w.x)/d{NAMES[k]:10s} FD {fd[k]: .9e}  fp64 adjoint (frozen) {dthz[k]: .9e} "
+            f"rel {abs(dthz[k] - fd[k]) / abs(fd[k]):.1e}  gpu {dth[k]: .9e} rel {abs(dth[k] - fd[k]) / abs(fd[k]):.1e}"
+            f"  (fp64 adjoint on the gpu trajectory: rel {abs(dth64[k] - fd[k]) / abs(fd[k]):.1e})")
+        assert abs(dthz[k] - fd[k]) <= 1e-5 * abs(fd[k])
+    gerr = np.linalg.norm(dth - fd) / np.linalg.norm(fd)
+    say(f"[adjoint]   contacts: |gpu - FD| / |FD| over the five parameters {gerr:.1e}")
+    assert gerr <= 1e-3
+    r = np.random.default_rng(9).standard_normal((sc.n, 3))
+    hv = 1e-4
+    fdv = float(np.sum(w * (frozen(THETA0, hv * r) - frozen(THETA0, -hv * r)))) / (2 * hv)
+    say(f"[adjoint]   contacts: dL/dv0 . r  FD {fdv: .9e}  fp64 adjoint (frozen) {float(np.sum(dv0z * r)): .9e}"
+        f"  gpu {float(np.sum(dv0 * r)): .9e}")
+    assert abs(float(np.sum(dv0z * r)) - fdv) <= 1e-5 * abs(fdv)
+    assert abs(float(np.sum(dv0 * r)) - fdv) <= 1e-3 * abs(fdv)
+    adj.close()
+    sim2.close()

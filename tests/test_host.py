@@ -10,6 +10,9 @@ import oracle as O
 from simulator_amd import _lib as L
 from simulator_amd import scene as S
 
+# every test here runs in the CPU suite and again in the GPU suite
+pytestmark = pytest.mark.usefixtures("both_suites")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 

@@ -15,6 +15,9 @@ from simulator_amd import _lib as L
 from simulator_amd import scene as S
 from simulator_amd.simulator import Mesh, ObjLoader, X, Y, Z
 
+# every test here runs in the CPU suite and again in the GPU suite
+pytestmark = pytest.mark.usefixtures("both_suites")
+
 REF = "/root/reference/resources"
 U = np.uint32(0xFFFFFFFF)        # atoi("") - 1 stored unsigned
 
