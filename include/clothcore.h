@@ -335,6 +335,22 @@ int cc_bvh_export(cc_sim* sim, uint64_t* keys, uint32_t* order, int32_t* childA,
 int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, uint8_t* inside,
                       float* out_pos, float* out_n);
 
+/* cc_collide_points without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under stream capture, serving loops).  Every pointer is DEVICE memory the
+   caller owns: pts3 (3n floats) in; hit (n), inside (n), out_pos (3n),
+   out_n (3n) out, with the meanings of cc_collide_points; status (may be
+   NULL) receives the query's traversal faults (0 = none; bit 0: a stack
+   overflow, that query taken as a miss).  Asynchronous on stream (NULL: the
+   simulator's stream); it reads the body pose of the simulator's current
+   frame, so order it after the steps it follows and before the next
+   cc_sim_step.  At most 65536 points per call (CC_ERR_INVALID beyond): the
+   traversal spill area is a fixed 15.7 MB array per GPU, held until the
+   process ends and shared by every call on that GPU -- calls that overlap
+   on one GPU race. */
+int cc_collide_points_noalloc(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, uint8_t* inside,
+                              float* out_pos, float* out_n, uint32_t* status, void* stream);
+
 /* ------------------------------------------------------------------- vbo */
 
 /* cuda_update_vbo (simulation.cu:196-231) into a device buffer with the

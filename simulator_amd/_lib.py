@@ -131,6 +131,7 @@ PROTOTYPES = {
     "cc_bvh_width": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "cc_bvh_export": (C.c_int, [P, P, P, P, P, P, P, P, P, P]),
     "cc_collide_points": (C.c_int, [P, P, u32, P, P, P, P]),
+    "cc_collide_points_noalloc": (C.c_int, [P, P, u32, P, P, P, P, P, P]),
     "cc_sim_set_faces": (C.c_int, [P, P, u32]),
     "cc_sim_write_vbo": (C.c_int, [P, P, C.c_size_t, u32, C.c_int]),
     "cc_sim_get_normals": (C.c_int, [P, P]),

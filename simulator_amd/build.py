@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libclothcore.so")
 
-SOURCES = ["host_scene.cpp", "host_obj.cpp", "lbvh.hip", "refit.hip", "sim.hip", "bbody.hip", "batch.hip", "adjoint.hip", "wide.hip"]
+SOURCES = ["host_scene.cpp", "host_obj.cpp", "lbvh.hip", "refit.hip", "sim.hip", "bbody.hip", "batch.hip", "adjoint.hip", "wide.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: a*b+c stays two roundings unless fmaf is written out;

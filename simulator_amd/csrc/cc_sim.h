@@ -383,6 +383,12 @@ struct Sim {
   }
 };
 
+// The collision query + projection of n points (collision_response_projection
+// without the Verlet part) over view v; spill: a spill area of n lanes whose
+// tail the caller has set.  Asynchronous on s.
+void launch_collide_points(const BvhView& v, const float* pts, uint32_t n, int32_t* spill, int32_t* hit,
+                           uint8_t* ins, float* out_pos, float* out_n, hipStream_t s);
+
 // All spring types of a cloth merged into one CSR row per vertex (types in
 // order, each row in its CSR order); records {end | type << 30, rest bits}.
 void merge_springs(const cc_cloth_desc* c, std::vector<uint32_t>& row, std::vector<uint2>& rec);

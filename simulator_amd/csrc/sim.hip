@@ -81,6 +81,13 @@ __global__ __launch_bounds__(256) void k_collide_points(BvhView bvh, const float
   st3(out_n + 3 * (size_t)i, nc);
 }
 
+void launch_collide_points(const BvhView& v, const float* pts, uint32_t n, int32_t* spill, int32_t* hit,
+                           uint8_t* ins, float* out_pos, float* out_n, hipStream_t s) {
+  if (n == 0) return;
+  k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(v, pts, n, spill, hit, ins, out_pos, out_n);
+  CC_HIP(hipGetLastError());
+}
+
 __global__ void k_finite(const float* __restrict__ s, uint32_t n, uint32_t* __restrict__ bad) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -737,8 +744,7 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     on.alloc(3 * (size_t)n);
     h.alloc(n);
     in.alloc(n);
-    k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(S.body.view(S.body.slot_of(S.frame)), p.p, n, sp.p, h.p, in.p, op.p, on.p);
-    CC_HIP(hipGetLastError());
+    launch_collide_points(S.body.view(S.body.slot_of(S.frame)), p.p, n, sp.p, h.p, in.p, op.p, on.p, s);
     CC_HIP(hipMemcpyAsync(hit, h.p, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(inside, in.p, n, hipMemcpyDeviceToHost, s));
     CC_HIP(hipMemcpyAsync(out_pos, op.p, 12 * (size_t)n, hipMemcpyDeviceToHost, s));
