@@ -253,7 +253,7 @@ struct Adj {
 
 static void check_symmetric(const Sim& S) {
   std::vector<uint2> rec(S.rec.n);
-  CC_HIP(hipMemcpy(rec.data(), S.rec.p, rec.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+  copy_on(rec.data(), S.rec.p, rec.size() * sizeof(uint2), hipMemcpyDeviceToHost, S.stream);
   const std::vector<uint32_t>& row = S.h_row;
   for (uint32_t i = 0; i < S.n; ++i)
     for (uint32_t k = row[i] & ROW_MASK; k < (row[i + 1] & ROW_MASK); ++k) {

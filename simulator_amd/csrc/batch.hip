@@ -307,7 +307,7 @@ int cc_batch_create(const cc_cloth_desc* cloths, const cc_body_desc* bodies, con
       B.ell.upload(er.data(), er.size(), s);
       B.ell_w = ELL_W;
     }
-    alloc_spill(B.spill, B.ntotal);
+    alloc_spill(B.spill, B.ntotal, s);
     // bodies: concatenated, frame 0 built for all of them
     B.bodies.init(bodies, params, count, s);
     std::vector<InstDesc> desc(count);
@@ -345,11 +345,11 @@ int cc_batch_set_pins(cc_batch* b, uint32_t inst, const uint32_t* idx, uint32_t 
     for (uint32_t q = 0; q < k; ++q) {
       B.h_row[o + idx[q]] |= PIN_BIT;
       float* dst = B.state[B.cur].p + 6 * (size_t)(o + idx[q]);
-      CC_HIP(hipMemcpy(rec6, dst, 24, hipMemcpyDeviceToHost));
+      copy_on(rec6, dst, 24, hipMemcpyDeviceToHost, b->b.stream);
       for (int c = 0; c < 3; ++c) rec6[3 + c] = rec6[c];
-      CC_HIP(hipMemcpy(dst, rec6, 24, hipMemcpyHostToDevice));
+      copy_on(dst, rec6, 24, hipMemcpyHostToDevice, b->b.stream);
     }
-    CC_HIP(hipMemcpy(B.row.p + o, B.h_row.data() + o, (size_t)n * 4, hipMemcpyHostToDevice));
+    copy_on(B.row.p + o, B.h_row.data() + o, (size_t)n * 4, hipMemcpyHostToDevice, b->b.stream);
   });
 }
 
@@ -385,8 +385,8 @@ int cc_batch_get_state(cc_batch* b, uint32_t inst, float* x_cur, float* x_prev, 
     CC_HIP(hipDeviceSynchronize());
     const uint32_t o = B.v_off[inst], n = B.v_off[inst + 1] - o;
     std::vector<float> st(6 * (size_t)n), c(3 * (size_t)n);
-    CC_HIP(hipMemcpy(st.data(), B.state[B.cur].p + 6 * (size_t)o, st.size() * 4, hipMemcpyDeviceToHost));
-    CC_HIP(hipMemcpy(c.data(), B.contact.p + 3 * (size_t)o, c.size() * 4, hipMemcpyDeviceToHost));
+    copy_on(st.data(), B.state[B.cur].p + 6 * (size_t)o, st.size() * 4, hipMemcpyDeviceToHost, b->b.stream);
+    copy_on(c.data(), B.contact.p + 3 * (size_t)o, c.size() * 4, hipMemcpyDeviceToHost, b->b.stream);
     for (uint32_t i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) {
         if (x_cur) x_cur[3 * (size_t)i + k] = st[6 * (size_t)i + k];

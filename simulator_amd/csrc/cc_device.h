@@ -50,11 +50,11 @@ __device__ inline void trav_fault(const Spill& s, uint32_t bit) {
 __device__ inline int stack_limit(const Spill& s) { return min(s.tail[1], STACK_LIMIT); }
 
 // host: a spill area for `lanes` lanes, its tail set to no faults and the
-// default stack limit (set-up only: a synchronous copy)
-inline void alloc_spill(DBuf<int32_t>& b, size_t lanes) {
+// default stack limit (set-up only: a synchronous copy on s)
+inline void alloc_spill(DBuf<int32_t>& b, size_t lanes, hipStream_t s) {
   b.alloc((size_t)STACK_SPILL * lanes + SPILL_TAIL);
   const int32_t tail[SPILL_TAIL] = {0, STACK_LIMIT};
-  CC_HIP(hipMemcpy(b.p + (size_t)STACK_SPILL * lanes, tail, sizeof(tail), hipMemcpyHostToDevice));
+  copy_on(b.p + (size_t)STACK_SPILL * lanes, tail, sizeof(tail), hipMemcpyHostToDevice, s);
 }
 
 // host: the finite check of n {x_cur, x_prev} records at `state` and the

@@ -91,6 +91,20 @@ struct DBuf {
 
 inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
+// Synchronous copy / fill on the object's own stream.  The library never
+// uses the null stream: created on first use, it takes one of the process's
+// hardware queues (four on MI355X by default) ahead of streams created later,
+// and an update lane created after it would then share a queue with the
+// step stream and be serialised behind it.
+inline void copy_on(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  CC_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  CC_HIP(hipStreamSynchronize(s));
+}
+inline void fill_on(void* dst, int value, size_t bytes, hipStream_t s) {
+  CC_HIP(hipMemsetAsync(dst, value, bytes, s));
+  CC_HIP(hipStreamSynchronize(s));
+}
+
 // ------------------------------------------------------------ physics pod
 struct Params {
   float dt, mass, gx, gy, gz, damp, kd, k0, k1, k2, rest_scale, eps, box_depth;

@@ -125,8 +125,9 @@ struct Sim {
 
   StepArgs args() const { return step_args(prm, ntypes); }
 
-  // the fused step reading body slot k and state[cur], writing state[cur ^ 1]
-  void launch_verlet(int k, hipStream_t s);
+  // the fused step reading body slot k and state[cur], writing state[cur ^ 1];
+  // e0 / e1 (optional): events stamped with the kernel's start and end
+  void launch_verlet(int k, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   // the plain fused step between caller buffers (no debug outputs, no
   // counters), with the given arguments; contact normals from c_in to c_out
   // (the same array for an update in place)
@@ -266,7 +267,9 @@ struct Sim {
         CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
         waited[l] = true;
       }
-      hipEvent_t* te = timing ? ring_next(1) : nullptr;
+      // updates are timed one frame in eight: event packets on the lanes
+      // cost the step several microseconds when every update carries them
+      hipEvent_t* te = (timing && g % 8 == 0) ? ring_next(1) : nullptr;
       if (te) CC_HIP(hipEventRecord(te[0], ls));
       body.update(k, l, g, ls);
       if (te) CC_HIP(hipEventRecord(te[1], ls));
@@ -298,9 +301,10 @@ struct Sim {
       CC_HIP(hipEventRecord(ev_go, s));
     }
     hipEvent_t* te = timing ? ring_next(0) : nullptr;
-    if (te) CC_HIP(hipEventRecord(te[0], s));
-    launch_verlet(k, s);
-    if (te) CC_HIP(hipEventRecord(te[1], s));
+    if (te)
+      launch_verlet(k, s, te[0], te[1]);
+    else
+      launch_verlet(k, s);
     if (pipe) issue_ahead(frame + (uint64_t)lookahead);
     cur ^= 1;
     ++frame;

@@ -29,9 +29,16 @@ constexpr int RC_CHUNK = 2048;
 constexpr int RC_THREADS = 512;
 constexpr int RC_PER_THREAD = RC_CHUNK / RC_THREADS;
 constexpr int TOP_THREADS = 1024;
-constexpr int TOP_REFIT_THREADS = 1024;
-constexpr int TOP_LOAD = 4;                                         // entries per thread per load round
-constexpr int TOP_MAX_LDS = 4096;                                   // entries held in LDS (36 B each)
+// The per-frame top pass is ONE workgroup of 256 threads with ~100 KB of
+// LDS (a single step block retiring leaves room for it on a CU).  Measured
+// at the 501k-triangle refitted body: the pipelined step is the same as with
+// 1024 threads and 147 KB (44.1-47.1 vs 43.5-48.4 us per step); in
+// isolation the pass takes ~25 us, of which loading the chunk results ~6,
+// the 30 levels ~10.5 and storing the boxes into the four-wide records ~8.5.
+constexpr int TOP_REFIT_THREADS = 256;
+constexpr int TOP_LOAD = 7;                                         // entries per thread per load round
+constexpr int TOP_MAX_LDS = 3200;                                   // entries held in LDS (32 B each)
+constexpr int TOP_PER_THREAD = (TOP_MAX_LDS + TOP_REFIT_THREADS - 1) / TOP_REFIT_THREADS;
 constexpr int TOP_MAX_LEVELS = 64;                                  // top-tree height bound (depth < 64)
 constexpr uint8_t BELOW_TOP = 0x80;                                 // flag: parent is a top node
 constexpr uint8_t IS_A = 0x40;                                      // flag: child A of the parent
@@ -394,11 +401,12 @@ __global__ __launch_bounds__(TOP_THREADS) void k_top_sort(int32_t* __restrict__ 
 
 // Per refit, one workgroup: entry e's box = sentinel U its non-top children
 // (top_child) U its top children (lower levels); one level per barrier, one
-// entry per thread within a level.
+// entry per thread within a level.  LDS holds the boxes and child links;
+// the slot a box goes to is read again from `links` when it is stored.
 template <bool LDS>
 __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t H, const uint32_t* __restrict__ lev,
                                  const int4* __restrict__ links, const float* __restrict__ top_child,
-                                 float* __restrict__ root_out, float* box, int2* lnk, int32_t* lsl,
+                                 float* __restrict__ root_out, float* box, int2* lnk,
                                  float4* __restrict__ wide, const uint2* __restrict__ wdst) {
   const uint32_t tid = threadIdx.x;
   // Entry boxes from the non-top children's boxes.  TOP_LOAD entries per
@@ -427,7 +435,6 @@ __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t 
       if (L[u].y < 0) unite(mn, mx, b);
       put6(box + 6 * (size_t)i, mn, mx);
       lnk[i] = make_int2(L[u].x, L[u].y);
-      lsl[i] = L[u].w;
     }
   }
   __syncthreads();
@@ -443,14 +450,33 @@ __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t 
     }
     __syncthreads();
   }
-  for (uint32_t i = tid; i < T; i += TOP_REFIT_THREADS) {
-    const int32_t s = lsl[i];
-    const float* b = box + 6 * (size_t)i;
-    if (wide && s >= 0) {
-      put_wide_at(wide, wdst[s], b);
-    } else {
-      float* d = s < 0 ? root_out : (s & 1 ? nodes[s >> 1].b_min : nodes[s >> 1].a_min);
-      for (int c = 0; c < 6; ++c) d[c] = b[c];
+  // stores: every slot, then every wide target, of the thread's entries
+  // loaded before the first store (two memory round trips in all; the LDS
+  // form holds at most TOP_PER_THREAD entries per thread)
+  constexpr int R = LDS ? TOP_PER_THREAD : TOP_LOAD;
+  for (uint32_t i0 = 0; i0 < T; i0 += TOP_REFIT_THREADS * R) {
+    int32_t sl[R];
+    uint2 wd[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const uint32_t i = min(i0 + u * TOP_REFIT_THREADS + tid, T - 1);
+      sl[u] = links[i].w;
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+      wd[u] = (wide && sl[u] >= 0) ? wdst[sl[u]] : make_uint2(~0u, 0u);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const uint32_t i = i0 + u * TOP_REFIT_THREADS + tid;
+      if (i >= T) continue;
+      const int32_t s = sl[u];
+      const float* b = box + 6 * (size_t)i;
+      if (wide && s >= 0) {
+        put_wide_at(wide, wd[u], b);
+      } else {
+        float* d = s < 0 ? root_out : (s & 1 ? nodes[s >> 1].b_min : nodes[s >> 1].a_min);
+        for (int c = 0; c < 6; ++c) d[c] = b[c];
+      }
     }
   }
 }
@@ -466,21 +492,19 @@ __global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restric
                                                                  const uint2* __restrict__ wdst) {
   __shared__ float sbox[TOP_MAX_LDS * 6];
   __shared__ int2 slnk[TOP_MAX_LDS];
-  __shared__ int32_t ssl[TOP_MAX_LDS];
   __shared__ uint32_t slev[TOP_MAX_LEVELS + 2];
   const uint32_t T = *top_count, H = *top_hmax;
   if (T == 0) return;
   for (uint32_t h = threadIdx.x; h <= H + 1; h += TOP_REFIT_THREADS) slev[h] = lev[h];
   __syncthreads();
   if (T <= (uint32_t)TOP_MAX_LDS) {
-    refit_top_levels<true>(nodes, T, H, slev, links, top_child, root_out, sbox, slnk, ssl, wide, wdst);
+    refit_top_levels<true>(nodes, T, H, slev, links, top_child, root_out, sbox, slnk, wide, wdst);
   } else {
     // more entries than LDS holds: the same levels on global scratch (one
     // workgroup, so the barriers order it)
     float* b = scratch;
     int2* l = reinterpret_cast<int2*>(b + 6 * (size_t)T);
-    int32_t* sl = reinterpret_cast<int32_t*>(l + T);
-    refit_top_levels<false>(nodes, T, H, slev, links, top_child, root_out, b, l, sl, wide, wdst);
+    refit_top_levels<false>(nodes, T, H, slev, links, top_child, root_out, b, l, wide, wdst);
   }
 }
 

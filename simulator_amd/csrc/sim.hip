@@ -15,6 +15,8 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "cc_step.h"
 
 namespace cc {
@@ -162,48 +164,65 @@ StepArgs step_args(const cc_params& prm, uint32_t ntypes) {
 }
 
 // ---------------------------------------------------------- Sim launches
+// One launch of a step kernel; with events, through hipExtLaunchKernel,
+// which stamps e0 / e1 with the kernel's own start and end (no marker
+// packets between the kernels of the step stream).
+template <class K>
+static void launch_one(K kern, unsigned g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const float* in, float* out,
+                       const float* c_in, float* c_out, const uint32_t* row, const uint2* rec, const StepArgs& a,
+                       const BvhView& v, uint32_t n, int32_t* spill, float* dbg_pre, int32_t* dbg_hit,
+                       uint8_t* dbg_in, unsigned long long* ctr) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, e0, e1, 0, in, out, c_in, c_out, row, rec, a, v, n, spill,
+                          dbg_pre, dbg_hit, dbg_in, ctr);
+  else
+    kern<<<g, 256, 0, s>>>(in, out, c_in, c_out, row, rec, a, v, n, spill, dbg_pre, dbg_hit, dbg_in, ctr);
+}
+
 template <bool DEBUG, bool COUNT>
 static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float* in, float* out, const float* contact_in, float* contact_out,
                             const StepArgs& a, const BvhView& v, float* dbg_pre, int32_t* dbg_hit, uint8_t* dbg_in,
-                            unsigned long long* ctr) {
+                            unsigned long long* ctr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  const uint2* recs = S.ell_w ? S.ell.p : S.rec.p;
+#define CC_STEP_ARGS g, s, e0, e1, in, out, contact_in, contact_out, S.row.p, recs, a, v, S.n, S.spill.p, dbg_pre, \
+                     dbg_hit, dbg_in, ctr
   if (v.wide) {
     if (S.ell_w == ELL_W)
-      k_verlet<DEBUG, COUNT, ELL_W, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
-                                                            S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, ELL_W, true>, CC_STEP_ARGS);
     else if (S.ell_w == ELL_W2)
-      k_verlet<DEBUG, COUNT, ELL_W2, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
-                                                             S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, ELL_W2, true>, CC_STEP_ARGS);
     else
-      k_verlet<DEBUG, COUNT, 0, true><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n,
-                                                        S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, 0, true>, CC_STEP_ARGS);
   } else {
     if (S.ell_w == ELL_W)
-      k_verlet<DEBUG, COUNT, ELL_W, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
-                                                             S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, ELL_W, false>, CC_STEP_ARGS);
     else if (S.ell_w == ELL_W2)
-      k_verlet<DEBUG, COUNT, ELL_W2, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.ell.p, a, v,
-                                                              S.n, S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, ELL_W2, false>, CC_STEP_ARGS);
     else
-      k_verlet<DEBUG, COUNT, 0, false><<<g, 256, 0, s>>>(in, out, contact_in, contact_out, S.row.p, S.rec.p, a, v, S.n,
-                                                         S.spill.p, dbg_pre, dbg_hit, dbg_in, ctr);
+      launch_one(k_verlet<DEBUG, COUNT, 0, false>, CC_STEP_ARGS);
   }
+#undef CC_STEP_ARGS
   CC_HIP(hipGetLastError());
 }
 
-void Sim::launch_verlet(int k, hipStream_t s) {
+void Sim::launch_verlet(int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const float* in = state[cur].p;
   float* out = state[cur ^ 1].p;
   const BvhView v = body.view(k);
   const unsigned g = grid_for(n, 256);
   const StepArgs a = args();
   if (debug && counting)
-    launch_k_verlet<true, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<true, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p,
+                                e0, e1);
   else if (debug)
-    launch_k_verlet<true, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<true, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p,
+                                 ctr.p, e0, e1);
   else if (counting)
-    launch_k_verlet<false, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<false, true>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p,
+                                 ctr.p, e0, e1);
   else
-    launch_k_verlet<false, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p, ctr.p);
+    launch_k_verlet<false, false>(*this, g, s, in, out, contact.p, contact.p, a, v, dbg_pre.p, dbg_hit.p, dbg_in.p,
+                                  ctr.p, e0, e1);
 }
 
 void Sim::launch_step(const float* in, float* out, const float* c_in, float* c_out, const StepArgs& a, int k,
@@ -355,7 +374,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       S.ell.upload(er.data(), er.size(), s);
       S.ell_w = (int)w;
     }
-    alloc_spill(S.spill, S.n);
+    alloc_spill(S.spill, S.n, s);
     S.state[0].upload(packed.data(), packed.size(), s);
     S.state[1].alloc(packed.size());
     S.state[1].zero(s);
@@ -387,8 +406,10 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       mp.py = bd->pivot[1];
       mp.pz = bd->pivot[2];
       mp.dt = (double)params->dt;
-      // three update lanes: measured 2% faster than two at the 501k-triangle
-      // refitted body, four 17% slower (more updates contend with the step)
+      // three update lanes: at the 501k-triangle refitted body, A/B in one
+      // process per setting on the same GPU, 44.6-44.8 us per step in graph
+      // blocks (frames 50-530) against 49.3-49.5 with two and 52.6-53.5
+      // with four; one call per frame 52.4-52.7 against 52.2-52.4 with two
       int lanes = 3;
       if (const char* e = std::getenv("CC_BODY_LANES")) lanes = std::atoi(e);
       CC_REQUIRE(lanes >= 1 && lanes <= MAX_LANES, "cc_sim_create: CC_BODY_LANES must be 1..4");
@@ -454,13 +475,13 @@ int cc_sim_get_state(cc_sim* sim, float* x_cur, float* x_prev, float* contact_n)
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
     std::vector<float> packed(6 * (size_t)S.n);
-    CC_HIP(hipMemcpy(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost));
+    copy_on(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost, sim->s.stream);
     for (uint32_t i = 0; i < S.n; ++i)
       for (int c = 0; c < 3; ++c) {
         if (x_cur) x_cur[3 * (size_t)i + c] = packed[6 * (size_t)i + c];
         if (x_prev) x_prev[3 * (size_t)i + c] = packed[6 * (size_t)i + 3 + c];
       }
-    if (contact_n) CC_HIP(hipMemcpy(contact_n, S.contact.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost));
+    if (contact_n) copy_on(contact_n, S.contact.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost, sim->s.stream);
   });
 }
 
@@ -470,14 +491,14 @@ int cc_sim_set_state(cc_sim* sim, const float* x_cur, const float* x_prev, const
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
     std::vector<float> packed(6 * (size_t)S.n);
-    CC_HIP(hipMemcpy(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost));
+    copy_on(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost, sim->s.stream);
     for (uint32_t i = 0; i < S.n; ++i)
       for (int c = 0; c < 3; ++c) {
         if (x_cur) packed[6 * (size_t)i + c] = x_cur[3 * (size_t)i + c];
         if (x_prev) packed[6 * (size_t)i + 3 + c] = x_prev[3 * (size_t)i + c];
       }
-    CC_HIP(hipMemcpy(S.state[S.cur].p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice));
-    if (contact_n) CC_HIP(hipMemcpy(S.contact.p, contact_n, 3 * (size_t)S.n * 4, hipMemcpyHostToDevice));
+    copy_on(S.state[S.cur].p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, sim->s.stream);
+    if (contact_n) copy_on(S.contact.p, contact_n, 3 * (size_t)S.n * 4, hipMemcpyHostToDevice, sim->s.stream);
   });
 }
 
@@ -511,7 +532,7 @@ int cc_sim_update_vertex(cc_sim* sim, uint32_t idx, const float xyz[3]) {
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
     float rec6[6] = {xyz[0], xyz[1], xyz[2], xyz[0], xyz[1], xyz[2]};
-    CC_HIP(hipMemcpy(S.state[S.cur].p + 6 * (size_t)idx, rec6, sizeof(rec6), hipMemcpyHostToDevice));
+    copy_on(S.state[S.cur].p + 6 * (size_t)idx, rec6, sizeof(rec6), hipMemcpyHostToDevice, sim->s.stream);
   });
 }
 
@@ -526,11 +547,11 @@ int cc_sim_set_pins(cc_sim* sim, const uint32_t* idx, uint32_t k) {
     for (uint32_t q = 0; q < k; ++q) {
       S.h_row[idx[q]] |= PIN_BIT;
       float* dst = S.state[S.cur].p + 6 * (size_t)idx[q];
-      CC_HIP(hipMemcpy(rec6.data(), dst, 24, hipMemcpyDeviceToHost));
+      copy_on(rec6.data(), dst, 24, hipMemcpyDeviceToHost, sim->s.stream);
       for (int c = 0; c < 3; ++c) rec6[3 + c] = rec6[c];
-      CC_HIP(hipMemcpy(dst, rec6.data(), 24, hipMemcpyHostToDevice));
+      copy_on(dst, rec6.data(), 24, hipMemcpyHostToDevice, sim->s.stream);
     }
-    CC_HIP(hipMemcpy(S.row.p, S.h_row.data(), S.h_row.size() * 4, hipMemcpyHostToDevice));
+    copy_on(S.row.p, S.h_row.data(), S.h_row.size() * 4, hipMemcpyHostToDevice, sim->s.stream);
   });
 }
 
@@ -566,9 +587,9 @@ int cc_sim_get_debug(cc_sim* sim, float* pre3, int32_t* hit, uint8_t* inside) {
     CC_REQUIRE(sim && sim->s.debug, "cc_sim_get_debug: debug outputs not enabled");
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
-    if (pre3) CC_HIP(hipMemcpy(pre3, S.dbg_pre.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost));
-    if (hit) CC_HIP(hipMemcpy(hit, S.dbg_hit.p, (size_t)S.n * 4, hipMemcpyDeviceToHost));
-    if (inside) CC_HIP(hipMemcpy(inside, S.dbg_in.p, S.n, hipMemcpyDeviceToHost));
+    if (pre3) copy_on(pre3, S.dbg_pre.p, 3 * (size_t)S.n * 4, hipMemcpyDeviceToHost, sim->s.stream);
+    if (hit) copy_on(hit, S.dbg_hit.p, (size_t)S.n * 4, hipMemcpyDeviceToHost, sim->s.stream);
+    if (inside) copy_on(inside, S.dbg_in.p, S.n, hipMemcpyDeviceToHost, sim->s.stream);
   });
 }
 
@@ -577,7 +598,7 @@ int cc_sim_set_counting(cc_sim* sim, int enable) {
     CC_REQUIRE(sim, "cc_sim_set_counting: null");
     CC_HIP(hipDeviceSynchronize());
     sim->s.counting = enable != 0;
-    CC_HIP(hipMemset(sim->s.ctr.p, 0, sim->s.ctr.n * sizeof(unsigned long long)));
+    fill_on(sim->s.ctr.p, 0, sim->s.ctr.n * sizeof(unsigned long long), sim->s.stream);
     sim->s.drop_graphs();
   });
 }
@@ -587,7 +608,7 @@ int cc_sim_get_counters(cc_sim* sim, uint64_t* nodes_visited, uint64_t* queries)
     CC_REQUIRE(sim, "cc_sim_get_counters: null");
     CC_HIP(hipDeviceSynchronize());
     std::vector<unsigned long long> c(sim->s.ctr.n);
-    CC_HIP(hipMemcpy(c.data(), sim->s.ctr.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    copy_on(c.data(), sim->s.ctr.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, sim->s.stream);
     unsigned long long v = 0, q = 0;
     for (size_t b = 0; b + 1 < c.size(); b += 2) {
       v += c[b];
@@ -613,7 +634,7 @@ int cc_sim_set_stack_limit(cc_sim* sim, int limit) {
     Sim& S = sim->s;
     CC_HIP(hipDeviceSynchronize());
     const int32_t v = limit;
-    CC_HIP(hipMemcpy(S.spill.p + (size_t)STACK_SPILL * S.n + 1, &v, 4, hipMemcpyHostToDevice));
+    copy_on(S.spill.p + (size_t)STACK_SPILL * S.n + 1, &v, 4, hipMemcpyHostToDevice, sim->s.stream);
   });
 }
 
@@ -625,8 +646,8 @@ int cc_body_set_vertices(cc_sim* sim, const float* verts3) {
     CC_HIP(hipDeviceSynchronize());
     Sim& S = sim->s;
     const int k = B.slot_of(S.frame);
-    CC_HIP(hipMemcpy(B.rest.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
-    CC_HIP(hipMemcpy(B.slot[k].verts.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice));
+    copy_on(B.rest.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice, sim->s.stream);
+    copy_on(B.slot[k].verts.p, verts3, 3 * (size_t)B.nv * 4, hipMemcpyHostToDevice, sim->s.stream);
     // new rest pose: the topology from it as given, then the current frame's pose
     B.build(k, 0, S.stream);
     B.share_topology(k, S.stream);
@@ -640,7 +661,7 @@ int cc_body_get_vertices(cc_sim* sim, float* verts3) {
     Body& B = sim->s.body;
     CC_REQUIRE(B.mode != CC_BODY_NONE, "cc_body_get_vertices: no body");
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemcpy(verts3, B.slot[B.slot_of(sim->s.frame)].verts.p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost));
+    copy_on(verts3, B.slot[B.slot_of(sim->s.frame)].verts.p, 3 * (size_t)B.nv * 4, hipMemcpyDeviceToHost, sim->s.stream);
   });
 }
 
@@ -651,7 +672,7 @@ int cc_body_get_motion(cc_sim* sim, float* R, float* T, float* breath) {
     CC_REQUIRE(B.ncaps > 0, "cc_body_get_motion: body has no motion");
     CC_HIP(hipDeviceSynchronize());
     std::vector<float> m(12 + B.ncaps);
-    CC_HIP(hipMemcpy(m.data(), B.slot[B.slot_of(sim->s.frame)].motion.p, m.size() * 4, hipMemcpyDeviceToHost));
+    copy_on(m.data(), B.slot[B.slot_of(sim->s.frame)].motion.p, m.size() * 4, hipMemcpyDeviceToHost, sim->s.stream);
     if (R) std::memcpy(R, m.data(), 36);
     if (T) std::memcpy(T, m.data() + 9, 12);
     if (breath) std::memcpy(breath, m.data() + 12, 4 * (size_t)B.ncaps);
@@ -739,7 +760,7 @@ int cc_collide_points(cc_sim* sim, const float* pts3, uint32_t n, int32_t* hit, 
     DBuf<uint8_t> in;
     DBuf<int32_t> sp;
     p.upload(pts3, 3 * (size_t)n, s);
-    alloc_spill(sp, n);
+    alloc_spill(sp, n, s);
     op.alloc(3 * (size_t)n);
     on.alloc(3 * (size_t)n);
     h.alloc(n);
@@ -780,7 +801,7 @@ int cc_sim_pick(cc_sim* sim, const float origin[3], const float dir[3], int32_t*
       const uint32_t b = (ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord;
       std::memcpy(&tt, &b, 4);
       uint32_t v0 = 0;     // OnMouseDown selects the face's first vertex (scene.cpp:431-434)
-      CC_HIP(hipMemcpy(&v0, S.faces.p + 3 * (size_t)f, 4, hipMemcpyDeviceToHost));
+      copy_on(&v0, S.faces.p + 3 * (size_t)f, 4, hipMemcpyDeviceToHost, sim->s.stream);
       v = (int32_t)v0;
     }
     if (face) *face = f;
@@ -840,7 +861,7 @@ int cc_sim_get_normals(cc_sim* sim, float* normals3) {
   return guard([&] {
     CC_REQUIRE(sim && normals3 && sim->s.nf > 0, "cc_sim_get_normals: bad argument");
     CC_HIP(hipDeviceSynchronize());
-    CC_HIP(hipMemcpy(normals3, sim->s.vnormals.p, 12 * (size_t)sim->s.n, hipMemcpyDeviceToHost));
+    copy_on(normals3, sim->s.vnormals.p, 12 * (size_t)sim->s.n, hipMemcpyDeviceToHost, sim->s.stream);
   });
 }
 
