@@ -15,6 +15,8 @@
 namespace cc {
 
 constexpr int MAX_CAPS = 64;
+// leaves per workgroup of the refit's chunk pass (refit.hip)
+constexpr int REFIT_CHUNK = 2048;
 constexpr int MAX_SLOTS = 64;
 constexpr int MAX_LANES = 4;
 

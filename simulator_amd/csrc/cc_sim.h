@@ -91,6 +91,23 @@ struct Sim {
   hipEvent_t ev_pose[MAX_SLOTS] = {};
   bool pose_pending[MAX_SLOTS] = {};
   bool block_inflight = false;
+  // "go" (ev_go) is recorded on the step stream before the step of frame
+  // go_frame, so it completes once the steps before that frame are done.  A
+  // lane about to overwrite the slot of frame g needs the step of frame
+  // g - nslots done, i.e. a go recorded at go_frame > g - nslots: single
+  // steps record a new go only every GO_EVERY frames and a lane waits on it
+  // only when the one it last waited on (lane_go) no longer covers its next
+  // frame -- each cross-stream wait or event record is a runtime call of a
+  // few microseconds on the host's per-frame path.
+  static constexpr int GO_EVERY = 8;
+  uint64_t go_frame = 0;
+  bool go_valid = false;
+  uint64_t lane_go[MAX_LANES] = {};
+  bool lane_go_valid[MAX_LANES] = {};
+  // the last event recorded on each lane, and whether lane work is pending
+  // that the step stream has not joined
+  hipEvent_t lane_last[MAX_LANES] = {};
+  bool lane_pending[MAX_LANES] = {};
   // Timing of single steps and of their updates: a ring of event pairs on
   // the streams they bracket, read once the ring is full or on collection.
   static constexpr int TRING = 256;
@@ -143,7 +160,14 @@ struct Sim {
   // every event has completed): the next step starts the pipeline afresh.
   void reset_pipeline() {
     for (bool& p : pose_pending) p = false;
-    lanes_inflight = block_inflight = false;
+    for (int l = 0; l < MAX_LANES; ++l) lane_pending[l] = lane_go_valid[l] = false;
+    lanes_inflight = block_inflight = go_valid = false;
+  }
+
+  void record_go(hipStream_t s) {
+    CC_HIP(hipEventRecord(ev_go, s));
+    go_frame = frame;
+    go_valid = true;
   }
 
   // Moving body: drop the poses computed ahead and recompute the current
@@ -220,7 +244,11 @@ struct Sim {
   // the step stream waits for all lane work issued so far
   void join_lanes(hipStream_t s) {
     if (!lanes_inflight) return;
-    for (int l = 0; l < body.nlanes; ++l) CC_HIP(hipStreamWaitEvent(s, ev_done[l], 0));
+    for (int l = 0; l < body.nlanes; ++l)
+      if (lane_pending[l]) {
+        CC_HIP(hipStreamWaitEvent(s, lane_last[l], 0));
+        lane_pending[l] = false;
+      }
     lanes_inflight = block_inflight = false;
     for (bool& p : pose_pending) p = false;
   }
@@ -253,19 +281,19 @@ struct Sim {
   }
 
   // Lane work of the single-step path: the poses of frames [ready, upto]
-  // on lanes g % nlanes.  Each waits for "go", recorded on the step stream
-  // before the current step: the slot of frame g was last read by the step
-  // of frame g - nslots, earlier than the current one (upto < frame + nslots).
+  // on lanes g % nlanes, each after a go that covers its slot (see
+  // GO_EVERY; the caller has recorded one at most GO_EVERY frames ago, and
+  // upto + GO_EVERY < frame + nslots).
   void issue_ahead(uint64_t upto) {
-    bool waited[MAX_LANES] = {};
     for (; ready <= upto; ++ready) {
       const uint64_t g = ready;
       const int l = (int)(g % (uint64_t)body.nlanes);
       const int k = body.slot_of(g);
       hipStream_t ls = lane_s[l];
-      if (!waited[l]) {
+      if (!lane_go_valid[l] || lane_go[l] + (uint64_t)body.nslots <= g) {
         CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
-        waited[l] = true;
+        lane_go[l] = go_frame;
+        lane_go_valid[l] = true;
       }
       // updates are timed one frame in eight: event packets on the lanes
       // cost the step several microseconds when every update carries them
@@ -274,7 +302,8 @@ struct Sim {
       body.update(k, l, g, ls);
       if (te) CC_HIP(hipEventRecord(te[1], ls));
       CC_HIP(hipEventRecord(ev_pose[k], ls));
-      CC_HIP(hipEventRecord(ev_done[l], ls));
+      lane_last[l] = ev_pose[k];
+      lane_pending[l] = true;
       pose_pending[k] = true;
       lanes_inflight = true;
     }
@@ -298,7 +327,7 @@ struct Sim {
         CC_HIP(hipStreamWaitEvent(s, ev_pose[k], 0));
         pose_pending[k] = false;
       }
-      CC_HIP(hipEventRecord(ev_go, s));
+      if (!go_valid || frame >= go_frame + GO_EVERY) record_go(s);
     }
     hipEvent_t* te = timing ? ring_next(0) : nullptr;
     if (te)
@@ -332,11 +361,11 @@ struct Sim {
       // poses of this block not yet issued: on the lanes, in parallel (their
       // slots were last read by steps before this one), then joined
       if (ready < frame + G) {
-        CC_HIP(hipEventRecord(ev_go, s));
+        record_go(s);
         issue_ahead(frame + G - 1);
       }
       join_lanes(s);
-      CC_HIP(hipEventRecord(ev_go, s));
+      record_go(s);
     }
     // The step graph is submitted first: submitting a graph to a stream that
     // waits on another stream can hold the host until that wait is over,
@@ -352,6 +381,8 @@ struct Sim {
       for (int l = 0; l < body.nlanes; ++l) {
         hipStream_t ls = lane_s[l];
         CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
+        lane_go[l] = go_frame;
+        lane_go_valid[l] = true;
         if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
         for (int j = l; j < G; j += body.nlanes) {
           const uint64_t f = frame + (uint64_t)(G + j);
@@ -359,6 +390,8 @@ struct Sim {
         }
         if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
         CC_HIP(hipEventRecord(ev_done[l], ls));
+        lane_last[l] = ev_done[l];
+        lane_pending[l] = true;
       }
       lanes_inflight = block_inflight = true;
       ready = frame + 2 * G;

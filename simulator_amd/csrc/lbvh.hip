@@ -340,7 +340,7 @@ void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
   nflag.zero(s);
   lflag.alloc(std::max<uint32_t>(m, 1));
   lflag.zero(s);
-  chunk_h.alloc(grid_for(std::max<uint32_t>(m, 1), 2048));
+  chunk_h.alloc(grid_for(std::max<uint32_t>(m, 1), REFIT_CHUNK));
   chunk_h.zero(s);
 }
 

@@ -25,7 +25,7 @@
 
 namespace cc {
 
-constexpr int RC_CHUNK = 2048;
+constexpr int RC_CHUNK = REFIT_CHUNK;
 constexpr int RC_THREADS = 512;
 constexpr int RC_PER_THREAD = RC_CHUNK / RC_THREADS;
 constexpr int TOP_THREADS = 1024;

@@ -437,8 +437,8 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         // single steps keep two updates per lane in flight
         S.lookahead = 2 * S.body.nlanes;
         if (const char* e = std::getenv("CC_LOOKAHEAD")) S.lookahead = std::atoi(e);
-        CC_REQUIRE(S.lookahead >= 1 && S.lookahead < S.body.nslots,
-                   "cc_sim_create: CC_LOOKAHEAD must be 1..slots-1");
+        CC_REQUIRE(S.lookahead >= 1 && S.lookahead + Sim::GO_EVERY + 1 < S.body.nslots,
+                   "cc_sim_create: CC_LOOKAHEAD too large for the slot ring");
       }
     }
     CC_HIP(hipStreamSynchronize(s));
