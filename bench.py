@@ -166,11 +166,12 @@ def cpu_baseline(sc, budget_s=15.0, nthreads=1, label="large"):
                       f"{el:.2f} s per run"}
 
 
-def batched_leg(warmup=16, steps=160, plain_steps=32):
+def batched_leg(warmup=16, steps=160, plain_steps=48):
     """The batched workload (256 instances, bodies rebuilt every frame) through
     one Batch, beside the plain per-instance loop (one Simulator per instance
-    advanced in turn, no synchronisation between instances) on the same GPU;
-    plain_steps=0 times the batch alone."""
+    advanced in turn, each call plain_steps frames -- whole graph blocks of
+    the simulator -- with no synchronisation between instances) on the same
+    GPU; plain_steps=0 times the batch alone."""
     from simulator_amd import batch as B
     from simulator_amd.simulator import Simulator
     scs = B.instances(256, seed=0)
@@ -183,17 +184,23 @@ def batched_leg(warmup=16, steps=160, plain_steps=32):
     b.synchronize()
     tb = time.perf_counter() - t0
     nv, tile, order = b.info()
+    tuned, fused = b.tuning()
     b.close()
     out = {"workload": "256 instances, cloth side log-uniform [16,192] (seed 0), capsule bodies of "
                        "2,112-36,096 triangles rebuilt every frame, dt=1e-3",
            "instances": len(scs), "vertices": n, "body_triangles": int(sum(sc.body.tris.shape[0] for sc in scs)),
            "value": n * steps / tb, "unit": "vertex-steps/s", "ms_per_step": 1e3 * tb / steps, "steps": steps,
-           "tile": tile, "order": "largest-first" if order else "instance"}
+           "tile": tile, "order": "largest-first" if order else "instance",
+           "tile_ran_in": "the per-body sort's launch (k_bb_sort_step)" if fused else "k_verlet_batch",
+           "tuner_ms_per_step": {f"tile{t}/{o}": round(float(tuned[i, j]), 4)
+                                 for i, t in enumerate((64, 128, 256))
+                                 for j, o in enumerate(("instance", "largest-first"))}}
     if plain_steps <= 0:
         return out
     sims = [Simulator.from_scene(sc) for sc in scs]
+    block = sims[0].block_steps
     for s in sims:
-        s.simulate(16)
+        s.simulate(block)
     for s in sims:
         s.synchronize()
     t0 = time.perf_counter()
@@ -206,7 +213,10 @@ def batched_leg(warmup=16, steps=160, plain_steps=32):
         s.close()
     vp = n * plain_steps / tp
     out.update({"plain_loop_value": vp, "plain_loop_ms_per_step": 1e3 * tp / plain_steps,
-                "plain_loop_steps": plain_steps, "speedup_vs_plain": out["value"] / vp})
+                "plain_loop_steps": plain_steps,
+                "plain_loop_composition": "per simulator: %d graph blocks of %d steps + %d single steps" % (
+                    plain_steps // block, block, plain_steps % block),
+                "speedup_vs_plain": out["value"] / vp})
     return out
 
 

@@ -437,9 +437,13 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
    and body, advanced together: one launch per step covers every instance,
    runs of 16 steps replay a captured hipGraph.  The reference runs one
    Simulator per scene (simulator.h:9-59); a batch equals count separate
-   cc_sim handles stepped in turn, bit for bit.  The workgroup size and the
-   order of the workgroups are picked at creation by timing each candidate
-   (cc_batch_info reports the choice).  bodies may be NULL (no colliders) or
+   cc_sim handles stepped in turn, bit for bit.  The step's tile (vertices
+   of one instance per block entry: 64, 128 or 256) and the order of the
+   entries (instance order, or largest instance first) are picked at
+   creation by timing each candidate on the launch the steps use -- the
+   step kernel, or with moving bodies the per-body sort's launch that
+   carries the step (cc_batch_info reports the choice, cc_batch_tuning the
+   timings).  bodies may be NULL (no colliders) or
    hold count entries of any mode: REFIT / REBUILD bodies are posed (and
    refitted / rebuilt) at every frame for all instances together before the
    cloths step, as cc_sim_step does for one body. */
@@ -465,6 +469,10 @@ int cc_batch_frame(cc_batch* b, uint64_t* frame);
 /* Total vertices, the chosen workgroup size (64, 128 or 256) and order
    (0: instance order, 1: largest instance first); any pointer may be NULL. */
 int cc_batch_info(cc_batch* b, uint32_t* vertices, uint32_t* tile, uint32_t* order);
+/* The tuner's timings in ms per step, ms[2 * t + o] for tile 64 / 128 / 256
+   (t = 0, 1, 2) and order o; fused = 1 when they timed the sort's launch
+   carrying the step (moving bodies).  Either pointer may be NULL. */
+int cc_batch_tuning(cc_batch* b, float ms[6], uint32_t* fused);
 
 #ifdef __cplusplus
 }

@@ -157,6 +157,7 @@ PROTOTYPES = {
     "cc_batch_set_pins": (C.c_int, [P, C.c_uint32, P, C.c_uint32]),
     "cc_batch_get_state": (C.c_int, [P, C.c_uint32, P, P, P]),
     "cc_batch_info": (C.c_int, [P, P, P, P]),
+    "cc_batch_tuning": (C.c_int, [P, P, P]),
     "cc_batch_body_export": (C.c_int, [P, C.c_uint32, P, P, P, P, P, P, P, P, P, P]),
     "cc_batch_frame": (C.c_int, [P, P]),
     "cc_sim_get_timing": (C.c_int, [P, C.POINTER(cc_timing)]),

@@ -135,6 +135,14 @@ class Batch:
         L.check(L.lib().cc_batch_get_state(self._h, int(inst), L.ptr(x), L.ptr(xp), L.ptr(c)))
         return x, xp, c
 
+    def tuning(self):
+        """The tuner's ms per step for tiles 64 / 128 / 256 x order 0 / 1
+        (a (3, 2) array) and whether it timed the fused sort + step launch."""
+        ms = np.zeros(6, np.float32)
+        fused = C.c_uint32()
+        L.check(L.lib().cc_batch_tuning(self._h, L.ptr(ms), C.byref(fused)))
+        return ms.reshape(3, 2), bool(fused.value)
+
     def info(self):
         """(total vertices, workgroup size, workgroup order)."""
         v, t, o = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -59,8 +59,11 @@ struct Batch {
   int tile = 256, order = 0;
   DBuf<uint4> blocks;
   uint32_t nblocks = 0;
-  DBuf<uint4> blocks256;                     // 256-vertex tiles in the chosen order (fused with the sort)
-  uint32_t nblocks256 = 0;
+  // moving bodies: the step runs inside the sort's launch (k_bb_sort_step)
+  // with its own tile, tuned on that launch
+  int fused_tile = 256;
+  DBuf<uint4> fblocks;
+  uint32_t nfblocks = 0;
   int cur = 0;
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
 
@@ -107,8 +110,17 @@ struct Batch {
       launch_t<256>(blk, nblk, s);
   }
 
+  StepLaunch fused_launch(const uint4* blk, uint32_t nblk, int T) const {
+    return StepLaunch{state[cur].p, state[cur ^ 1].p, contact.p, row.p, ell_w ? ell.p : rec.p, insts[cur].p,
+                      blk, nblk, ntotal, spill.p, (uint32_t)ell_w, (uint32_t)T};
+  }
+
   // Time every (tile, order) candidate over a few steps from the initial
-  // state, keep the fastest, and restore the initial state.
+  // state on the launch the steps will use -- the plain step kernel, or,
+  // with moving bodies, the sort's launch that carries the step -- keep the
+  // fastest, and restore the initial state.  (The sort in the timed fused
+  // launches re-sorts the set-up's boxes into the next slot, which every
+  // step rebuilds before reading.)
   void autotune() {
     const size_t n6 = 6 * (size_t)ntotal, n3 = 3 * (size_t)ntotal;
     DBuf<float> s0, c0;
@@ -119,25 +131,28 @@ struct Batch {
     hipEvent_t e0, e1;
     CC_HIP(hipEventCreate(&e0));
     CC_HIP(hipEventCreate(&e1));
+    const bool fused = bodies.moving && bodies.step.n_sort;
     float best = 1e30f;
     for (int T : TILES)
       for (int ord = 0; ord < 2; ++ord) {
         const std::vector<uint4> t = block_table(T, ord);
         DBuf<uint4> blk;
         blk.upload(t.data(), t.size(), stream);
-        for (int k = 0; k < 3; ++k) {          // warm-up
-          launch(T, blk.p, (uint32_t)t.size(), stream);
+        auto run = [&] {
+          if (fused)
+            bodies.sort_with_step(bodies.step, cur ^ 1, fused_launch(blk.p, (uint32_t)t.size(), T), stream);
+          else
+            launch(T, blk.p, (uint32_t)t.size(), stream);
           cur ^= 1;
-        }
+        };
+        for (int k = 0; k < 3; ++k) run();     // warm-up
         CC_HIP(hipEventRecord(e0, stream));
-        for (int k = 0; k < 10; ++k) {
-          launch(T, blk.p, (uint32_t)t.size(), stream);
-          cur ^= 1;
-        }
+        for (int k = 0; k < 10; ++k) run();
         CC_HIP(hipEventRecord(e1, stream));
         CC_HIP(hipEventSynchronize(e1));
         float ms = 0;
         CC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        tuned[T == 64 ? 0 : T == 128 ? 1 : 2][ord] = ms / 10.0f;
         if (ms < best) {
           best = ms;
           tile = T;
@@ -149,14 +164,20 @@ struct Batch {
     cur = 0;
     CC_HIP(hipMemcpyAsync(state[0].p, s0.p, n6 * sizeof(float), hipMemcpyDeviceToDevice, stream));
     CC_HIP(hipMemcpyAsync(contact.p, c0.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    // the set-up's frame-0 bodies again in both slots the timed sorts wrote
+    if (fused) {
+      bodies.launch(bodies.all, 0, 0, stream);
+      bodies.launch(bodies.all, 0, 1, stream);
+    }
     const std::vector<uint4> t = block_table(tile, order);
     blocks.upload(t.data(), t.size(), stream);
     nblocks = (uint32_t)t.size();
-    const std::vector<uint4> t256 = block_table(256, order);
-    blocks256.upload(t256.data(), t256.size(), stream);
-    nblocks256 = (uint32_t)t256.size();
+    fused_tile = tile;
+    fblocks.upload(t.data(), t.size(), stream);
+    nfblocks = (uint32_t)t.size();
     CC_HIP(hipStreamSynchronize(stream));
   }
+  float tuned[3][2] = {};                    // ms per step of each (tile, order) candidate
 
   // One step of frame `frame` (cur == frame % 2): the moving bodies of frame
   // + 1 are posed, sorted and built into the other slot while the cloths step
@@ -166,9 +187,7 @@ struct Batch {
     if (bodies.moving) {
       bodies.pre(bodies.step, k, nxt, s);
       if (bodies.step.n_sort) {
-        StepLaunch st{state[cur].p, state[cur ^ 1].p, contact.p, row.p, ell_w ? ell.p : rec.p, insts[cur].p,
-                      blocks256.p, nblocks256, ntotal, spill.p, (uint32_t)ell_w};
-        bodies.sort_with_step(bodies.step, nxt, st, s);
+        bodies.sort_with_step(bodies.step, nxt, fused_launch(fblocks.p, nfblocks, fused_tile), s);
       } else {
         launch(tile, blocks.p, nblocks, s);
       }
@@ -416,6 +435,17 @@ int cc_batch_frame(cc_batch* b, uint64_t* frame) {
   return guard([&] {
     CC_REQUIRE(b && frame, "cc_batch_frame: bad argument");
     *frame = b->b.frame;
+  });
+}
+
+int cc_batch_tuning(cc_batch* b, float ms[6], uint32_t* fused) {
+  return guard([&] {
+    CC_REQUIRE(b, "cc_batch_tuning: null");
+    const Batch& B = b->b;
+    if (ms)
+      for (int t = 0; t < 3; ++t)
+        for (int o = 0; o < 2; ++o) ms[2 * t + o] = B.tuned[t][o];
+    if (fused) *fused = (B.bodies.moving && B.bodies.step.n_sort) ? 1u : 0u;
   });
 }
 

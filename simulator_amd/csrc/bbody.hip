@@ -279,10 +279,10 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort(const BodyDesc* __restri
   bb_sort(list[blockIdx.x], desc, boxes, partial, kv0, kv1, keys, leaf_tri, tris, root);
 }
 
-// The sort's workgroups, then one cloth step of the batch in blocks of four
-// 256-vertex tiles (the sort of frame g + 1 and the step of frame g read and
-// write disjoint buffers).
-template <int EW>
+// The sort's workgroups, then one cloth step of the batch in workgroups of
+// SB_THREADS / TILE tiles of TILE vertices (the sort of frame g + 1 and the
+// step of frame g read and write disjoint buffers).
+template <int EW, int TILE>
 __global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __restrict__ desc,
                                                              const uint32_t* __restrict__ list, uint32_t nsort,
                                                              const float* __restrict__ boxes,
@@ -295,10 +295,10 @@ __global__ __launch_bounds__(SB_THREADS) void k_bb_sort_step(const BodyDesc* __r
     bb_sort(list[blockIdx.x], desc, boxes, partial, kv0, kv1, keys, leaf_tri, tris, root);
     return;
   }
-  const uint32_t e = (blockIdx.x - nsort) * (SB_THREADS / 256) + (threadIdx.x >> 8);
+  const uint32_t e = (blockIdx.x - nsort) * (SB_THREADS / TILE) + threadIdx.x / TILE;
   if (e >= st.nblocks) return;
   const uint4 b = st.blocks[e];
-  const uint32_t lt = threadIdx.x & 255u;
+  const uint32_t lt = threadIdx.x % TILE;
   if (lt >= b.z) return;
   const InstDesc dd = st.insts[b.x];
   const uint32_t i = b.y + lt;
@@ -819,16 +819,26 @@ void BatchBodies::sort(const BodyPass& P, int slot, hipStream_t s) {
   CC_HIP(hipGetLastError());
 }
 
+template <int EW, int TILE>
+static void launch_sort_step(BatchBodies& B, const BodyPass& P, int slot, const StepLaunch& st, hipStream_t s) {
+  const uint32_t per = SB_THREADS / TILE;
+  const uint32_t nstep = (st.nblocks + per - 1) / per;
+  k_bb_sort_step<EW, TILE><<<P.n_sort + nstep, SB_THREADS, 0, s>>>(
+      B.desc.p, P.sort.p, P.n_sort, B.boxes.p, B.partial.p, B.kv[0].p, B.kv[1].p, B.keys.p, B.leaf_tri[slot].p,
+      B.tris.p, B.root[slot].p, st);
+}
+
 void BatchBodies::sort_with_step(const BodyPass& P, int slot, const StepLaunch& st, hipStream_t s) {
-  const uint32_t nstep = (st.nblocks + SB_THREADS / 256 - 1) / (SB_THREADS / 256);
-  if (st.ew == ELL_W)
-    k_bb_sort_step<ELL_W><<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p,
-                                                                  kv[0].p, kv[1].p, keys.p, leaf_tri[slot].p, tris.p,
-                                                                  root[slot].p, st);
-  else
-    k_bb_sort_step<0><<<P.n_sort + nstep, SB_THREADS, 0, s>>>(desc.p, P.sort.p, P.n_sort, boxes.p, partial.p,
-                                                              kv[0].p, kv[1].p, keys.p, leaf_tri[slot].p, tris.p,
-                                                              root[slot].p, st);
+  CC_REQUIRE(st.tile == 64 || st.tile == 128 || st.tile == 256, "batch: step tile must be 64, 128 or 256");
+  if (st.ew == ELL_W) {
+    if (st.tile == 64) launch_sort_step<ELL_W, 64>(*this, P, slot, st, s);
+    else if (st.tile == 128) launch_sort_step<ELL_W, 128>(*this, P, slot, st, s);
+    else launch_sort_step<ELL_W, 256>(*this, P, slot, st, s);
+  } else {
+    if (st.tile == 64) launch_sort_step<0, 64>(*this, P, slot, st, s);
+    else if (st.tile == 128) launch_sort_step<0, 128>(*this, P, slot, st, s);
+    else launch_sort_step<0, 256>(*this, P, slot, st, s);
+  }
   CC_HIP(hipGetLastError());
 }
 

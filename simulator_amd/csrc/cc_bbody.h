@@ -41,6 +41,7 @@ struct StepLaunch {
   uint32_t ntotal;
   int32_t* spill;
   uint32_t ew;              // rec holds ew records per vertex (0: merged rows at row)
+  uint32_t tile;            // vertices per block entry (64, 128 or 256)
 };
 
 struct BatchBodies {

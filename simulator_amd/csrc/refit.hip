@@ -260,10 +260,12 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     }
     if (x >= c1) lf[r] = 0;
   }
-  // Round 2: the leaves' vertices, and the top-pass record of every parent
-  // that a box goes to directly (the parent's top position).
+  // Round 2: the leaves' vertices, and for every box that goes directly to
+  // a top node, that node's top-pass record and (four-wide form) the wide
+  // entry of its slot.
   float vtx[RC_PER_THREAD][9];
   int32_t tpos_n[RC_PER_THREAD], tpos_l[RC_PER_THREAD];
+  uint2 wtop_n[RC_PER_THREAD], wtop_l[RC_PER_THREAD];
 #pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const uint32_t t3[3] = {tri[r].x, tri[r].y, tri[r].z};
@@ -271,8 +273,11 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int c = 0; c < 3; ++c) vtx[r][3 * k + c] = verts[3 * (size_t)t3[k] + c];
-    tpos_n[r] = top_pos[min(max(L[r].z, 0), tmax)];
-    tpos_l[r] = top_pos[min(max(lpar[r], 0), tmax)];
+    const int32_t pn = min(max(L[r].z, 0), tmax), pl = min(max(lpar[r], 0), tmax);
+    tpos_n[r] = top_pos[pn];
+    tpos_l[r] = top_pos[pl];
+    wtop_n[r] = wide ? wdst[2 * (size_t)pn + ((fl[r] & IS_A) ? 0 : 1)] : make_uint2(~0u, 0u);
+    wtop_l[r] = wide ? wdst[2 * (size_t)pl + ((lf[r] & IS_A) ? 0 : 1)] : make_uint2(~0u, 0u);
   }
 #pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
@@ -306,6 +311,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
   }
   // stores: both child slots of every in-chunk node; boxes of nodes / leaves
   // directly below a top node to that node's slot and its top-pass record
+#pragma unroll
   for (int r = 0; r < RC_PER_THREAD; ++r) {
     const int32_t x = c0 + r * RC_THREADS + (int32_t)tid;
     if (x < i1 && (fl[r] & H_MASK)) {
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
         } else {
           const bool isA = (fl[r] & IS_A) != 0;
           if (wide)
-            put_wide_at(wide, wdst[2 * (size_t)p + (isA ? 0 : 1)], bx);
+            put_wide_at(wide, wtop_n[r], bx);
           else
             put6(isA ? nodes[p].a_min : nodes[p].b_min, bx, bx + 3);
           put6(top_child + 12 * (size_t)tpos_n[r] + (isA ? 0 : 6), bx, bx + 3);
@@ -340,7 +346,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
       const int32_t p = lpar[r];
       const bool isA = (lf[r] & IS_A) != 0;
       if (wide)
-        put_wide_at(wide, wdst[2 * (size_t)p + (isA ? 0 : 1)], lbox[l - c0]);
+        put_wide_at(wide, wtop_l[r], lbox[l - c0]);
       else
         put6(isA ? nodes[p].a_min : nodes[p].b_min, lbox[l - c0], lbox[l - c0] + 3);
       put6(top_child + 12 * (size_t)tpos_l[r] + (isA ? 0 : 6), lbox[l - c0], lbox[l - c0] + 3);
