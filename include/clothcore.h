@@ -395,10 +395,13 @@ typedef struct cc_timing {
 int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
 
 /* Timing inside the normal path: while enabled, cc_sim_step records HIP
-   events around each graph of fused step kernels, around each single fused
-   step kernel, and around each body update lane's launches, on the streams
-   they run on, and accumulates their durations (the host reads the events
-   of earlier blocks and steps while later ones run).
+   events around each graph of fused step kernels, around the single fused
+   step kernels of frames divisible by four, and around each block's body
+   update lanes and the single-step updates of frames divisible by eight, on
+   the streams they run on, and accumulates their durations (the host reads
+   the events of earlier blocks and steps while later ones run; a launch
+   that stamps events costs the host ~10 us more than a plain one, so
+   single steps are sampled).
    cc_sim_get_timing returns the averages since enabling: verlet_us per
    fused step kernel, refit_us (refitted body) or build_us (rebuilt body)
    per body update including its deformation, steps = steps timed.

@@ -97,8 +97,11 @@ struct Sim {
   // g - nslots done, i.e. a go recorded at go_frame > g - nslots: single
   // steps record a new go only every GO_EVERY frames and a lane waits on it
   // only when the one it last waited on (lane_go) no longer covers its next
-  // frame -- each cross-stream wait or event record is a runtime call of a
-  // few microseconds on the host's per-frame path.
+  // frame -- on MI355X a hipStreamWaitEvent on a pending event costs the
+  // host ~15 us and an event record ~1.4 us, on the per-frame path.  (Signal
+  // words with hipStreamWriteValue64 / hipStreamWaitValue64 cost the host
+  // less, but the pipelined steps ran slower with them: 38.5 against 37.2 us
+  // per single step, 54 against 51.5 us in contact.)
   static constexpr int GO_EVERY = 8;
   uint64_t go_frame = 0;
   bool go_valid = false;
@@ -329,7 +332,9 @@ struct Sim {
       }
       if (!go_valid || frame >= go_frame + GO_EVERY) record_go(s);
     }
-    hipEvent_t* te = timing ? ring_next(0) : nullptr;
+    // timed one step in four: a launch that stamps events costs the host
+    // ~10 us more than a plain one (12.8 against 2.9 us on MI355X)
+    hipEvent_t* te = (timing && frame % 4 == 0) ? ring_next(0) : nullptr;
     if (te)
       launch_verlet(k, s, te[0], te[1]);
     else
