@@ -410,6 +410,19 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);
 int cc_sim_set_timing(cc_sim* sim, int enable);
 int cc_sim_get_timing(cc_sim* sim, cc_timing* out);
 
+/* Fault injection, for tests of the error paths (no counterpart in the
+   reference, whose safe_cuda prints an error and continues, common.h:4-10).
+   After `count` further successful operations of the kind, the next one
+   fails as if the runtime had failed it, and the switch is off again:
+   CC_FAULT_ALLOC a device (or pinned host) allocation -> CC_ERR_ALLOC,
+   CC_FAULT_LAUNCH the check after a kernel launch -> CC_ERR_HIP (the
+   kernel itself was launched and runs).  count < 0 turns the kind off.
+   Process-wide.  A call that failed this way leaves its object usable
+   only for destruction (a failed creation frees what it had made). */
+#define CC_FAULT_ALLOC 1
+#define CC_FAULT_LAUNCH 2
+int cc_fault_inject(int kind, int64_t count);
+
 /* -------------------------------------------------------------- adjoint */
 
 /* Reverse-mode gradient of a rollout (no counterpart in the reference: the

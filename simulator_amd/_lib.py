@@ -19,6 +19,9 @@ CC_ERR_NONFINITE = 4
 CC_ERR_UNSUPPORTED = 5
 CC_ERR_TRAVERSAL = 6
 
+CC_FAULT_ALLOC = 1
+CC_FAULT_LAUNCH = 2
+
 CC_BODY_NONE = 0
 CC_BODY_STATIC = 1
 CC_BODY_REFIT = 2
@@ -105,6 +108,7 @@ PROTOTYPES = {
     "cc_sim_create": (C.c_int, [C.POINTER(cc_cloth_desc), C.POINTER(cc_body_desc),
                                 C.POINTER(cc_params), C.POINTER(P)]),
     "cc_sim_destroy": (C.c_int, [P]),
+    "cc_fault_inject": (C.c_int, [C.c_int, C.c_int64]),
     "cc_sim_step": (C.c_int, [P, C.c_int, P]),
     "cc_sim_synchronize": (C.c_int, [P]),
     "cc_sim_update_vertex": (C.c_int, [P, u32, P]),
@@ -202,6 +206,13 @@ def device_count():
     n = C.c_int(0)
     check(lib().cc_device_count(C.byref(n)))
     return n.value
+
+
+def fault_inject(kind, count):
+    """cc_fault_inject: fail the allocation (CC_FAULT_ALLOC) or launch check
+    (CC_FAULT_LAUNCH) after `count` further successful ones; count < 0 turns
+    the kind off."""
+    check(lib().cc_fault_inject(int(kind), int(count)))
 
 
 def default_params():

@@ -249,6 +249,9 @@ struct Adj {
   DBuf<float> seg_state, seg_contact;    // [ckpt+1][6n], [ckpt+1][3n]
   DBuf<float> lp[2], lq[2], lf, locp, locq;
   DBuf<double> acc, out;
+  ~Adj() {
+    if (S && S->stream) (void)hipStreamSynchronize(S->stream);   // before the buffers are freed
+  }
 };
 
 static void check_symmetric(const Sim& S) {
@@ -399,7 +402,7 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
         k_adj_force<<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, stj, A.seg_contact.p + j * n3,
                                                    A.seg_contact.p + (j + 1) * n3, A.lp[cur].p, A.lq[cur].p, A.aa,
                                                    A.lf.p, A.locp.p, A.locq.p, A.acc.p);
-        CC_HIP(hipGetLastError());
+        CC_LAUNCHED();
         if (S.ell_w == ELL_W)
           k_adj_springs<ELL_W><<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, S.ell.p, stj, A.lf.p, A.locp.p,
                                                               A.locq.p, A.aa, A.lp[cur ^ 1].p, A.lq[cur ^ 1].p,
@@ -407,12 +410,12 @@ int cc_adj_backward(cc_adj* adj, const float* dl_dx_final, double dl_dtheta[5], 
         else
           k_adj_springs<0><<<A.nblk, ADJ_THREADS, 0, s>>>(A.n, S.row.p, S.rec.p, stj, A.lf.p, A.locp.p, A.locq.p,
                                                           A.aa, A.lp[cur ^ 1].p, A.lq[cur ^ 1].p, A.acc.p);
-        CC_HIP(hipGetLastError());
+        CC_LAUNCHED();
         cur ^= 1;
       }
     }
     k_reduce_f64<<<1, 256, 0, s>>>(A.acc.p, A.nblk, A.out.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     std::vector<float> lp(n3), lq(n3);
     double g[NPARAM];
     CC_HIP(hipMemcpyAsync(lp.data(), A.lp[cur].p, n3 * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -68,6 +68,7 @@ struct Batch {
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
 
   ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);   // before the buffers are freed
     for (hipGraphExec_t& g : graphs)
       if (g) (void)hipGraphExecDestroy(g);
     if (stream) (void)hipStreamDestroy(stream);
@@ -99,7 +100,7 @@ struct Batch {
     else
       k_verlet_batch<T, 0><<<nblk, T, 0, s>>>(in, out, contact.p, row.p, rec.p, insts[cur].p, blk, ntotal,
                                               spill.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
   }
   void launch(int T, const uint4* blk, uint32_t nblk, hipStream_t s) {
     if (T == 64)

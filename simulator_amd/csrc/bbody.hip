@@ -594,7 +594,7 @@ void launch_build_levels(const BodyDesc* desc, const LvBlock* const* level, cons
     else
       k_bb_level<false><<<n_level[L], LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, in, cin, items[L & 1],
                                                      nitems[L & 1], nodes, leaf_parent, root, leaf0);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
   }
 }
 
@@ -732,6 +732,7 @@ void BatchBodies::init(const cc_body_desc* bodies, const cc_params* params, uint
   motion.alloc((size_t)BB_FRAMES * nb * BB_MF);
   motion.zero(s);
   for (int q = 0; q < 2; ++q) {
+    if (fault_now(FAULT_ALLOC)) throw Error(CC_ERR_ALLOC, "hipHostMalloc: injected failure (cc_fault_inject)");
     CC_HIP(hipHostMalloc((void**)&h_motion[q], (size_t)BB_FRAMES * nb * BB_MF * sizeof(float), hipHostMallocDefault));
     std::memset(h_motion[q], 0, (size_t)BB_FRAMES * nb * BB_MF * sizeof(float));
     CC_HIP(hipEventCreate(&ev_motion[q]));
@@ -809,14 +810,14 @@ void BatchBodies::pre(const BodyPass& P, int k, int slot, hipStream_t s) {
                                                  capp1.p, motion.p, nb, k);
   if (P.n_boxes)
     k_bb_boxes<<<P.n_boxes, BB_BLOCK, 0, s>>>(desc.p, P.boxes.p, verts[slot].p, tris.p, boxes.p, partial.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 void BatchBodies::sort(const BodyPass& P, int slot, hipStream_t s) {
   if (nb == 0 || !P.n_sort) return;
   k_bb_sort<<<P.n_sort, SB_THREADS, 0, s>>>(desc.p, P.sort.p, boxes.p, partial.p, kv[0].p, kv[1].p, keys.p,
                                              leaf_tri[slot].p, tris.p, root[slot].p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 template <int EW, int TILE>
@@ -839,7 +840,7 @@ void BatchBodies::sort_with_step(const BodyPass& P, int slot, const StepLaunch& 
     else if (st.tile == 128) launch_sort_step<0, 128>(*this, P, slot, st, s);
     else launch_sort_step<0, 256>(*this, P, slot, st, s);
   }
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 void BatchBodies::post(const BodyPass& P, int slot, hipStream_t s) {

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -31,6 +32,27 @@ void set_last_error(const std::string& msg);
       (void)hipGetLastError();                                                        \
       throw ::cc::Error(CC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     }                                                                                 \
+  } while (0)
+
+// ------------------------------------------------------- fault injection
+// cc_fault_inject: after `count` further successful operations of a kind,
+// the next one fails as the runtime would have failed it, and the switch is
+// off again.  Process-wide, one counter per kind (atomic: any thread).
+enum FaultKind { FAULT_ALLOC = 0, FAULT_LAUNCH = 1, FAULT_KINDS = 2 };
+inline std::atomic<int64_t> g_fault[FAULT_KINDS]{{-1}, {-1}};
+inline bool fault_now(int kind) {
+  int64_t c = g_fault[kind].load(std::memory_order_relaxed);
+  while (c >= 0)
+    if (g_fault[kind].compare_exchange_weak(c, c - 1)) return c == 0;
+  return false;
+}
+
+// The check after a kernel launch (and where a fault is injected).
+#define CC_LAUNCHED()                                                                       \
+  do {                                                                                      \
+    if (::cc::fault_now(::cc::FAULT_LAUNCH))                                                \
+      throw ::cc::Error(CC_ERR_HIP, "kernel launch: injected failure (cc_fault_inject)");  \
+    CC_HIP(hipGetLastError());                                                              \
   } while (0)
 
 #define CC_REQUIRE(cond, msg)                                       \
@@ -73,6 +95,7 @@ struct DBuf {
     if (count == n && p) return;
     release();
     if (count == 0) return;
+    if (fault_now(FAULT_ALLOC)) throw Error(CC_ERR_ALLOC, "hipMalloc: injected failure (cc_fault_inject)");
     hipError_t e = hipMalloc(&p, count * sizeof(T));
     if (e != hipSuccess) {
       p = nullptr;

@@ -118,7 +118,13 @@ struct Sim {
   uint8_t rkind[TRING] = {};     // 0: fused step, 1: body update
   uint64_t r_head = 0, r_tail = 0;
 
+  // Work still queued on the streams (a creation that failed part way, lane
+  // updates issued ahead of the last step) finishes before the buffers it
+  // uses are freed.
   ~Sim() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipStream_t ls : lane_s)
+      if (ls) (void)hipStreamSynchronize(ls);
     drop_graphs();
     for (auto& set : tev)
       for (hipEvent_t e : set)

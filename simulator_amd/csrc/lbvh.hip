@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ key
 void Body::prim_boxes_of(int k, int l, hipStream_t s) {
   Lane& L = lane[l];
   k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(slot[k].verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 void Body::build_levels(int k, int l, hipStream_t s) {
@@ -410,7 +410,7 @@ void Body::deform(int k, uint64_t frame, hipStream_t s) {
   const MotionFrame mf = motion_of(motion, h_phase.data(), frame);
   k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, slot[k].verts.p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
                                                mf, slot[k].motion.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 void Body::build(int k, int l, hipStream_t s) {
@@ -419,10 +419,10 @@ void Body::build(int k, int l, hipStream_t s) {
   CC_REQUIRE(L.full, "body: build on a refit-only lane");
   Slot& S = slot[k];
   k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(S.verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   k_morton<<<grid_for(m, 256), 256, 0, s>>>(L.boxes.p, m, L.partial.p, L.nblk_box, L.code[0].p, L.idx[0].p,
                                             S.root.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   const uint32_t nblk = grid_for(m, SORT_TILE);
   int src = 0;
   for (int pass = 0, shift = 0; pass < RADIX_PASSES; ++pass, shift += RADIX_BITS) {
@@ -430,12 +430,12 @@ void Body::build(int k, int l, hipStream_t s) {
     k_radix_rowscan<<<RADIX_DIGITS / 4, 256, 0, s>>>(L.hist.p, nblk, L.digit_total.p);
     k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p,
                                                   L.digit_total.p, nblk, L.code[src ^ 1].p, L.idx[src ^ 1].p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     src ^= 1;
   }
   k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(L.code[src].p, L.idx[src].p, tris.p, m, L.keys.p,
                                                  leaf_tri_of(k));
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   if (bottom_up) {                      // tree, parent links and boxes, bottom-up
     build_levels(k, l, s);
     return;
@@ -443,7 +443,7 @@ void Body::build(int k, int l, hipStream_t s) {
   if (m >= 2) {
     k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(L.keys.p, (int32_t)m, S.nodes.p, L.leaf_parent.p,
                                                   L.top_count.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
   }
   top_setup(k, l, s);
   nwide = 0;             // a new topology: the wide table follows the refit

@@ -86,11 +86,11 @@ extern "C" int cc_collide_points_noalloc(cc_sim* sim, const float* pts3, uint32_
       }
       int32_t* tail = spill + (size_t)STACK_SPILL * n;
       k_noalloc_tail<<<1, 64, 0, s>>>(tail);
-      CC_HIP(hipGetLastError());
+      CC_LAUNCHED();
       launch_collide_points(S.body.view(k), pts3, n, spill, hit, inside, out_pos, out_n, s);
       if (status) {
         k_noalloc_status<<<1, 64, 0, s>>>(tail, status);
-        CC_HIP(hipGetLastError());
+        CC_LAUNCHED();
       }
     } catch (...) {
       if (prev != S.device) (void)hipSetDevice(prev);

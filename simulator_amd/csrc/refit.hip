@@ -522,21 +522,21 @@ void Body::top_setup(int k, int l, hipStream_t s) {
   const unsigned nchunk = grid_for(m, RC_CHUNK);
   const Node* nd = slot[k].nodes.p;
   k_node_links<<<grid_for(m - 1, 256), 256, 0, s>>>(nd, m - 1, L.links.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   k_top_list<<<nchunk, RC_THREADS, 0, s>>>(nd, m, L.top_list.p, L.top_pos.p, L.top_count.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   k_chunk_heights<<<nchunk, RC_THREADS, 0, s>>>(nd, L.leaf_parent.p, m, L.top_pos.p, L.nflag.p, L.lflag.p,
                                                 L.chunk_h.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   if (m > (uint32_t)RC_CHUNK) {
     k_top_links<<<grid_for(m, 256), 256, 0, s>>>(nd, L.top_list.p, L.top_pos.p, L.top_count.p, L.top_links.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     k_top_heights<<<1, TOP_THREADS, 0, s>>>(L.top_links.p, L.top_count.p,
                                             reinterpret_cast<uint32_t*>(L.top_scratch.p), L.top_h.p, L.top_hmax.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     k_top_sort<<<1, TOP_THREADS, 0, s>>>(L.top_list.p, L.top_pos.p, L.top_links.p, L.top_h.p, L.top_count.p,
                                          L.top_hmax.p, L.top_lev.p, reinterpret_cast<int32_t*>(L.top_scratch.p));
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
   }
 }
 
@@ -558,12 +558,12 @@ void Body::refit(int k, int l, hipStream_t s, bool binary) {
                                                               T.links.p, S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
                                                               T.chunk_h.p, T.top_pos.p, L.top_child.p, S.root.p,
                                                               S.leaf0_box.p, W, w_dst.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   if (m > (uint32_t)RC_CHUNK) {
     k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(S.nodes.p, T.top_links.p, T.top_lev.p, L.top_child.p,
                                                 T.top_count.p, T.top_hmax.p, S.root.p, L.top_scratch.p, W,
                                                 w_dst.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
   }
 }
 

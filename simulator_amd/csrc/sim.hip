@@ -87,7 +87,7 @@ void launch_collide_points(const BvhView& v, const float* pts, uint32_t n, int32
                            uint8_t* ins, float* out_pos, float* out_n, hipStream_t s) {
   if (n == 0) return;
   k_collide_points<<<grid_for(n, 256), 256, 0, s>>>(v, pts, n, spill, hit, ins, out_pos, out_n);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 __global__ void k_finite(const float* __restrict__ s, uint32_t n, uint32_t* __restrict__ bad) {
@@ -127,7 +127,7 @@ void check_state(const float* state, uint32_t n, DBuf<int32_t>& spill, size_t la
                  hipStream_t s, const char* who) {
   CC_HIP(hipMemsetAsync(flag, 0, 4, s));
   if (n) k_finite<<<grid_for(n, 256), 256, 0, s>>>(state, n, flag);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   uint32_t bad = 0, fault = 0;
   int32_t* tail = spill.p + (size_t)STACK_SPILL * lanes;
   CC_HIP(hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, s));
@@ -202,7 +202,7 @@ static void launch_k_verlet(const Sim& S, unsigned g, hipStream_t s, const float
       launch_one(k_verlet<DEBUG, COUNT, 0, false>, CC_STEP_ARGS);
   }
 #undef CC_STEP_ARGS
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
 }
 
 void Sim::launch_verlet(int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -443,6 +443,13 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
     }
     CC_HIP(hipStreamSynchronize(s));
     *out = h.release();
+  });
+}
+
+int cc_fault_inject(int kind, int64_t count) {
+  return guard([&] {
+    CC_REQUIRE(kind == CC_FAULT_ALLOC || kind == CC_FAULT_LAUNCH, "cc_fault_inject: unknown kind");
+    g_fault[kind == CC_FAULT_ALLOC ? FAULT_ALLOC : FAULT_LAUNCH].store(count < 0 ? -1 : count);
   });
 }
 
@@ -789,7 +796,7 @@ int cc_sim_pick(cc_sim* sim, const float origin[3], const float dir[3], int32_t*
     CC_HIP(hipMemsetAsync(res.p, 0xFF, sizeof(unsigned long long), S.stream));
     const uint32_t g = std::min<uint32_t>(grid_for(S.nf, 256), 1024);
     k_pick<<<g, 256, 0, S.stream>>>(S.state[S.cur].p, S.faces.p, S.nf, ld3(origin), ld3(dir), res.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     unsigned long long key = 0;
     CC_HIP(hipMemcpyAsync(&key, res.p, sizeof(key), hipMemcpyDeviceToHost, S.stream));
     CC_HIP(hipStreamSynchronize(S.stream));
@@ -852,7 +859,7 @@ int cc_sim_write_vbo(cc_sim* sim, void* dst_dev, size_t bytes, uint32_t n_tex, i
     k_face_normals<<<grid_for(S.nf, 256), 256, 0, S.stream>>>(fn_state, S.faces.p, S.nf, S.face_n.p);
     k_vbo<<<grid_for(S.n, 256), 256, 0, S.stream>>>(pos_state, S.vrow.p, S.vcol.p, S.face_n.p, S.n, base, nrm,
                                                     S.vnormals.p);
-    CC_HIP(hipGetLastError());
+    CC_LAUNCHED();
     CC_HIP(hipStreamSynchronize(S.stream));
   });
 }

@@ -186,7 +186,7 @@ void Body::wide_setup(int k, hipStream_t s) {
   // Slot k's records in full (links, empty entries, boxes); the other slots
   // get theirs from share_topology, and every refit rewrites the boxes.
   k_fold_wide<<<grid_for(nwide, 256), 256, 0, s>>>(slot[k].nodes.p, w_src.p, w_link.p, nwide, slot[k].wide.p);
-  CC_HIP(hipGetLastError());
+  CC_LAUNCHED();
   CC_HIP(hipStreamSynchronize(s));
 }
 

@@ -8,10 +8,17 @@ faults, traps or hangs):
     recorded on the device and reported by the next check
     (CC_ERR_TRAVERSAL), once; at the default limit nothing is recorded and
     the results equal a simulator that never had its limit lowered;
-  * the batch's check passes after a run of every instance.
+  * the batch's check passes after a run of every instance;
+  * an allocation made to fail (cc_fault_inject) at any point of a creation
+    is reported as CC_ERR_ALLOC and the failed creation frees what it had
+    made; a launch check made to fail is reported as CC_ERR_HIP, the failed
+    simulator can be destroyed, and a fresh one reproduces an undisturbed
+    run bit for bit.
 
 The reference prints HIP errors and continues (common.h:4-10) and has no
 overflow test in its traversal (cudaBRT.cu:243-245)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -93,5 +100,90 @@ def test_batch_check_after_a_run(gpu):
     scs = B.instances(6, seed=4)
     b = B.Batch(scs)
     b.simulate(40)
+    b.check()
+    b.close()
+
+
+def _free_device_bytes():
+    # the HIP runtime that the library itself loaded (another copy of the
+    # runtime in the process would have its own state)
+    L.lib()
+    with open("/proc/self/maps") as f:
+        paths = sorted({line.split()[-1] for line in f if "libamdhip64" in line},
+                       key=lambda q: not q.startswith("/opt/rocm"))
+    path = paths[0]
+    hip = C.CDLL(path)
+    hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def _moving_scene():
+    body = S.capsules(64, 14, 6, L.CC_BODY_REFIT)
+    return S.grid_scene("moving", 24, 1e-3, 0, body, box_depth=0.002, y0=0.93)
+
+
+def _fail_nth(kind, k, make):
+    """make() with the k-th operation of `kind` failing; the object, or the
+    error it raised."""
+    L.fault_inject(kind, k)
+    try:
+        return make()
+    except L.ClothCoreError as e:
+        return e
+    finally:
+        L.fault_inject(kind, -1)
+
+
+def test_allocation_failure_at_every_point_of_a_creation(gpu):
+    sc = _moving_scene()
+    Simulator.from_scene(sc).close()          # runtime set-up outside the count
+    base = _free_device_bytes()
+    k = 0
+    while True:
+        r = _fail_nth(L.CC_FAULT_ALLOC, k, lambda: Simulator.from_scene(sc))
+        if not isinstance(r, L.ClothCoreError):
+            break
+        assert r.code == L.CC_ERR_ALLOC and "injected" in str(r), str(r)
+        k += 1
+        assert k < 2000, "the creation never completed"
+    say(f"[errors] a moving-body creation makes {k} device allocations; each one failed in turn")
+    sim = r
+    sim.simulate(30)                          # the creation that got through works
+    sim.check_finite()
+    sim.close()
+    lost = base - _free_device_bytes()
+    say(f"[errors] device memory after {k} failed creations: {lost / 2**20:.1f} MiB below the start")
+    assert lost < 64 * 2**20                  # a leaked creation would hold hundreds of MiB
+
+
+def test_launch_failure_is_reported_and_a_fresh_simulator_recovers(gpu):
+    sc = _moving_scene()
+    r = _fail_nth(L.CC_FAULT_LAUNCH, 0, lambda: Simulator.from_scene(sc))
+    assert isinstance(r, L.ClothCoreError) and r.code == L.CC_ERR_HIP and "injected" in str(r), r
+    ref = Simulator.from_scene(sc)
+    ref.simulate(40)
+    for k in (0, 5, 40):                      # the 1st, 6th and 41st launch check of the call
+        sim = Simulator.from_scene(sc)
+        sim.simulate(2)
+        r = _fail_nth(L.CC_FAULT_LAUNCH, k, lambda: sim.simulate(38))
+        assert isinstance(r, L.ClothCoreError) and r.code == L.CC_ERR_HIP, r
+        say(f"[errors] launch check {k} of a 38-step call failed: {r}")
+        sim.close()
+        again = Simulator.from_scene(sc)
+        again.simulate(40)
+        for a, b in zip(again.get_state(), ref.get_state()):
+            np.testing.assert_array_equal(a, b)
+        again.close()
+    ref.close()
+
+
+def test_batch_allocation_failure(gpu):
+    scs = B.instances(4, seed=2)
+    r = _fail_nth(L.CC_FAULT_ALLOC, 3, lambda: B.Batch(scs))
+    assert isinstance(r, L.ClothCoreError) and r.code == L.CC_ERR_ALLOC, r
+    b = B.Batch(scs)
+    b.simulate(10)
     b.check()
     b.close()

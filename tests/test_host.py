@@ -147,3 +147,14 @@ def test_vertex_extend_matches_oracle():
     np.testing.assert_array_equal(m.vertices, want)
     r = np.linalg.norm(m.vertices[:, :3] - np.float32([0.5, 0.5, 0.0]), axis=1)
     np.testing.assert_allclose(r, 0.303, rtol=1e-5)
+
+
+def test_fault_inject_arguments():
+    """cc_fault_inject takes CC_FAULT_ALLOC / CC_FAULT_LAUNCH and any count
+    (negative: off); other kinds are CC_ERR_INVALID.  No device work."""
+    lib = L.lib()
+    assert lib.cc_fault_inject(0, 1) == L.CC_ERR_INVALID
+    assert lib.cc_fault_inject(3, 1) == L.CC_ERR_INVALID
+    for kind in (L.CC_FAULT_ALLOC, L.CC_FAULT_LAUNCH):
+        assert lib.cc_fault_inject(kind, 5) == L.CC_OK
+        assert lib.cc_fault_inject(kind, -1) == L.CC_OK
