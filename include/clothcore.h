@@ -269,6 +269,17 @@ int cc_sim_set_params(cc_sim* sim, const cc_params* params);
 int cc_sim_get_state(cc_sim* sim, float* x_cur, float* x_prev, float* contact_n);
 int cc_sim_set_state(cc_sim* sim, const float* x_cur, const float* x_prev, const float* contact_n);
 int cc_sim_frame(cc_sim* sim, uint64_t* frame);
+/* Checkpoint of the cloth to a binary file (no counterpart in the
+   reference: Simulator::save is empty, simulation.cu:233-235).
+   cc_sim_save writes x_cur, x_last, the contact normals, the pinned
+   vertices, the parameters and the frame; cc_sim_load restores them into a
+   simulator of the same cloth (same vertex count and spring types), after
+   which it steps as the saved one would have.  CC_ERR_INVALID for another
+   cloth, a file that is not such a checkpoint, or one cut short.  The
+   body's pose follows from the frame (the rest pose and motion are the
+   simulator's creation inputs and are not saved). */
+int cc_sim_save(cc_sim* sim, const char* path);
+int cc_sim_load(cc_sim* sim, const char* path);
 int cc_sim_set_frame(cc_sim* sim, uint64_t frame);
 
 /* Debug outputs of the next steps: pre-projection positions (vec3), the

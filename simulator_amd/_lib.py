@@ -109,6 +109,8 @@ PROTOTYPES = {
                                 C.POINTER(cc_params), C.POINTER(P)]),
     "cc_sim_destroy": (C.c_int, [P]),
     "cc_fault_inject": (C.c_int, [C.c_int, C.c_int64]),
+    "cc_sim_save": (C.c_int, [P, C.c_char_p]),
+    "cc_sim_load": (C.c_int, [P, C.c_char_p]),
     "cc_sim_step": (C.c_int, [P, C.c_int, P]),
     "cc_sim_synchronize": (C.c_int, [P]),
     "cc_sim_update_vertex": (C.c_int, [P, u32, P]),

@@ -9,6 +9,7 @@
 //   row       4 B merged CSR row start of all spring types; bit 31 = pinned
 //   rec       8 B per directed spring {end | type << 30, rest length}
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cfloat>
 #include <cstring>
@@ -506,6 +507,94 @@ int cc_sim_set_state(cc_sim* sim, const float* x_cur, const float* x_prev, const
       }
     copy_on(S.state[S.cur].p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, sim->s.stream);
     if (contact_n) copy_on(S.contact.p, contact_n, 3 * (size_t)S.n * 4, hipMemcpyHostToDevice, sim->s.stream);
+  });
+}
+
+// Checkpoint file: header, pinned vertex indices, the state as stored
+// ({x_cur, x_last} per vertex), the contact normals.
+namespace {
+constexpr char CKPT_MAGIC[8] = {'C', 'C', 'S', 'I', 'M', 'C', 'K', 'P'};
+constexpr uint32_t CKPT_VERSION = 1;
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, n, ntypes, npins;
+  uint64_t frame;
+  cc_params prm;
+};
+struct File {
+  FILE* f = nullptr;
+  File(const char* path, const char* mode) : f(std::fopen(path, mode)) {}
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+int cc_sim_save(cc_sim* sim, const char* path) {
+  return guard([&] {
+    CC_REQUIRE(sim && path, "cc_sim_save: null argument");
+    Sim& S = sim->s;
+    CC_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> pins;
+    for (uint32_t i = 0; i < S.n; ++i)
+      if (S.h_row[i] & PIN_BIT) pins.push_back(i);
+    std::vector<float> packed(6 * (size_t)S.n), contact(3 * (size_t)S.n);
+    copy_on(packed.data(), S.state[S.cur].p, packed.size() * 4, hipMemcpyDeviceToHost, S.stream);
+    copy_on(contact.data(), S.contact.p, contact.size() * 4, hipMemcpyDeviceToHost, S.stream);
+    CkptHeader h{};
+    std::memcpy(h.magic, CKPT_MAGIC, sizeof(h.magic));
+    h.version = CKPT_VERSION;
+    h.n = S.n;
+    h.ntypes = S.ntypes;
+    h.npins = (uint32_t)pins.size();
+    h.frame = S.frame;
+    h.prm = S.prm;
+    File f(path, "wb");
+    CC_REQUIRE(f.f, std::string("cc_sim_save: cannot open ") + path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f.f) == 1;
+    ok = ok && (pins.empty() || std::fwrite(pins.data(), 4, pins.size(), f.f) == pins.size());
+    ok = ok && std::fwrite(packed.data(), 4, packed.size(), f.f) == packed.size();
+    ok = ok && std::fwrite(contact.data(), 4, contact.size(), f.f) == contact.size();
+    ok = ok && std::fclose(f.f) == 0;
+    f.f = nullptr;
+    CC_REQUIRE(ok, std::string("cc_sim_save: write failed: ") + path);
+  });
+}
+
+int cc_sim_load(cc_sim* sim, const char* path) {
+  return guard([&] {
+    CC_REQUIRE(sim && path, "cc_sim_load: null argument");
+    Sim& S = sim->s;
+    File f(path, "rb");
+    CC_REQUIRE(f.f, std::string("cc_sim_load: cannot open ") + path);
+    CkptHeader h{};
+    CC_REQUIRE(std::fread(&h, sizeof(h), 1, f.f) == 1 && std::memcmp(h.magic, CKPT_MAGIC, 8) == 0 &&
+                   h.version == CKPT_VERSION,
+               "cc_sim_load: not a cloth checkpoint of this version");
+    CC_REQUIRE(h.n == S.n && h.ntypes == S.ntypes,
+               "cc_sim_load: the checkpoint is of another cloth (vertex count or spring types differ)");
+    CC_REQUIRE(h.npins <= h.n && h.prm.dt > 0.0f && h.prm.mass > 0.0f, "cc_sim_load: corrupt checkpoint");
+    std::vector<uint32_t> pins(h.npins);
+    std::vector<float> packed(6 * (size_t)S.n), contact(3 * (size_t)S.n);
+    bool ok = pins.empty() || std::fread(pins.data(), 4, pins.size(), f.f) == pins.size();
+    ok = ok && std::fread(packed.data(), 4, packed.size(), f.f) == packed.size();
+    ok = ok && std::fread(contact.data(), 4, contact.size(), f.f) == contact.size();
+    CC_REQUIRE(ok, "cc_sim_load: truncated checkpoint");
+    for (uint32_t q : pins) CC_REQUIRE(q < S.n, "cc_sim_load: corrupt checkpoint (pin index)");
+    CC_HIP(hipDeviceSynchronize());
+    // parameters as cc_sim_set_params applies them, then pins, state, frame
+    S.prm = h.prm;
+    S.body.depth = h.prm.box_depth;
+    S.body.motion.dt = (double)h.prm.dt;
+    S.drop_graphs();
+    for (uint32_t i = 0; i < S.n; ++i) S.h_row[i] &= ROW_MASK;
+    for (uint32_t q : pins) S.h_row[q] |= PIN_BIT;
+    copy_on(S.row.p, S.h_row.data(), S.h_row.size() * 4, hipMemcpyHostToDevice, S.stream);
+    copy_on(S.state[S.cur].p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, S.stream);
+    copy_on(S.contact.p, contact.data(), contact.size() * 4, hipMemcpyHostToDevice, S.stream);
+    S.frame = h.frame;
+    S.ready = h.frame;
+    S.refresh_body();
   });
 }
 

@@ -8,6 +8,7 @@ Method names and argument meaning follow the reference; errors raise
 ClothCoreError instead of printing and continuing (common.h:4-10).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -284,6 +285,15 @@ class Simulator:
 
     def set_frame(self, frame):
         L.check(L.lib().cc_sim_set_frame(self._h, int(frame)))
+
+    def save(self, path):
+        """Checkpoint (cc_sim_save): state, contact normals, pins,
+        parameters and frame to a binary file."""
+        L.check(L.lib().cc_sim_save(self._h, os.fsencode(path)))
+
+    def load(self, path):
+        """Restore a checkpoint written by save() for this cloth (cc_sim_load)."""
+        L.check(L.lib().cc_sim_load(self._h, os.fsencode(path)))
 
     def set_debug(self, on=True):
         L.check(L.lib().cc_sim_set_debug(self._h, 1 if on else 0))

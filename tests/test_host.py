@@ -158,3 +158,11 @@ def test_fault_inject_arguments():
     for kind in (L.CC_FAULT_ALLOC, L.CC_FAULT_LAUNCH):
         assert lib.cc_fault_inject(kind, 5) == L.CC_OK
         assert lib.cc_fault_inject(kind, -1) == L.CC_OK
+
+
+def test_checkpoint_arguments():
+    """cc_sim_save / cc_sim_load refuse a null simulator or path (no device
+    work)."""
+    lib = L.lib()
+    assert lib.cc_sim_save(None, b"/tmp/x.ckpt") == L.CC_ERR_INVALID
+    assert lib.cc_sim_load(None, b"/tmp/x.ckpt") == L.CC_ERR_INVALID
