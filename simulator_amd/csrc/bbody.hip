@@ -360,14 +360,22 @@ __device__ inline void write_node(Node* N, int32_t idx, const float* bA, const f
 
 template <bool LEAVES>
 __global__ __launch_bounds__(LV_C) void k_bb_level(const BodyDesc* __restrict__ desc, const LvBlock* __restrict__ blk,
-                                                   const uint64_t* __restrict__ keys,
-                                                   const uint4* __restrict__ leaf_tri,
-                                                   const float* __restrict__ boxes,
-                                                   const BItem* __restrict__ items_in,
-                                                   const uint32_t* __restrict__ cnt_in, BItem* __restrict__ items_out,
-                                                   uint32_t* __restrict__ cnt_out, Node* __restrict__ nodes,
-                                                   int32_t* __restrict__ leaf_parent, float* __restrict__ root,
-                                                   float* __restrict__ leaf0) {
+                                                   PF<const uint64_t> keys_f, PF<const uint4> leaf_tri_f,
+                                                   PF<const float> boxes_f, PF<const BItem> items_in_f,
+                                                   PF<const uint32_t> cnt_in_f, PF<BItem> items_out_f,
+                                                   PF<uint32_t> cnt_out_f, PF<Node> nodes_f,
+                                                   PF<int32_t> leaf_parent_f, PF<float> root_f, PF<float> leaf0_f) {
+  const uint64_t* __restrict__ keys = pick(keys_f);
+  const uint4* __restrict__ leaf_tri = pick(leaf_tri_f);
+  const float* __restrict__ boxes = pick(boxes_f);
+  const BItem* __restrict__ items_in = pick(items_in_f);
+  const uint32_t* __restrict__ cnt_in = pick(cnt_in_f);
+  BItem* __restrict__ items_out = pick(items_out_f);
+  uint32_t* __restrict__ cnt_out = pick(cnt_out_f);
+  Node* __restrict__ nodes = pick(nodes_f);
+  int32_t* __restrict__ leaf_parent = pick(leaf_parent_f);
+  float* __restrict__ root = pick(root_f);
+  float* __restrict__ leaf0 = pick(leaf0_f);
   __shared__ int32_t sD[LV_C + 1];       // sD[k]: delta at the boundary left of item k; sD[n]: right of the last
   __shared__ uint32_t arrived[LV_C / 16];   // split k (2 bits at k % 16): +1 left child arrived, +2 right child
   __shared__ int32_t sR[LV_C];           // last leaf of item k
@@ -579,21 +587,44 @@ std::vector<std::vector<LvBlock>> build_level_tables(const std::vector<BodyDesc>
 }
 
 void launch_build_levels(const BodyDesc* desc, const LvBlock* const* level, const uint32_t* n_level,
-                         const uint64_t* keys, const uint4* leaf_tri, const float* boxes, BItem* items0,
-                         BItem* items1, uint32_t* nitems0, uint32_t* nitems1, Node* nodes, int32_t* leaf_parent,
-                         float* root, float* leaf0, hipStream_t s) {
-  BItem* items[2] = {items0, items1};
-  uint32_t* nitems[2] = {nitems0, nitems1};
+                         const LvFrame* frames, int nframes, hipStream_t s) {
+  CC_REQUIRE(nframes >= 1 && nframes <= MAX_FRAMES, "level build: frame count");
+  PF<const uint64_t> keys{};
+  PF<const uint4> leaf_tri{};
+  PF<const float> boxes{};
+  PF<BItem> items[2] = {};
+  PF<const BItem> items_c[2] = {};
+  PF<uint32_t> nitems[2] = {};
+  PF<const uint32_t> nitems_c[2] = {};
+  PF<Node> nodes{};
+  PF<int32_t> leaf_parent{};
+  PF<float> root{}, leaf0{};
+  for (int f = 0; f < nframes; ++f) {
+    const LvFrame& F = frames[f];
+    keys.p[f] = F.keys;
+    leaf_tri.p[f] = F.leaf_tri;
+    boxes.p[f] = F.boxes;
+    for (int q = 0; q < 2; ++q) {
+      items[q].p[f] = F.items[q];
+      items_c[q].p[f] = F.items[q];
+      nitems[q].p[f] = F.nitems[q];
+      nitems_c[q].p[f] = F.nitems[q];
+    }
+    nodes.p[f] = F.nodes;
+    leaf_parent.p[f] = F.leaf_parent;
+    root.p[f] = F.root;
+    leaf0.p[f] = F.leaf0;
+  }
   for (int L = 0; L < BB_MAX_LEVELS; ++L) {
     if (!n_level[L]) continue;
-    const BItem* in = items[(L + 1) & 1];
-    const uint32_t* cin = nitems[(L + 1) & 1];
+    const int in = (L + 1) & 1, out = L == 0 ? 0 : (L & 1);
+    const dim3 grid(n_level[L], (unsigned)nframes);
     if (L == 0)
-      k_bb_level<true><<<n_level[L], LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, in, cin, items[0],
-                                                    nitems[0], nodes, leaf_parent, root, leaf0);
+      k_bb_level<true><<<grid, LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, items_c[in], nitems_c[in],
+                                              items[out], nitems[out], nodes, leaf_parent, root, leaf0);
     else
-      k_bb_level<false><<<n_level[L], LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, in, cin, items[L & 1],
-                                                     nitems[L & 1], nodes, leaf_parent, root, leaf0);
+      k_bb_level<false><<<grid, LV_C, 0, s>>>(desc, level[L], keys, leaf_tri, boxes, items_c[in], nitems_c[in],
+                                               items[out], nitems[out], nodes, leaf_parent, root, leaf0);
     CC_LAUNCHED();
   }
 }
@@ -847,8 +878,9 @@ void BatchBodies::post(const BodyPass& P, int slot, hipStream_t s) {
   if (nb == 0) return;
   const LvBlock* tabs[BB_MAX_LEVELS];
   for (int L = 0; L < BB_MAX_LEVELS; ++L) tabs[L] = P.level[L].p;
-  launch_build_levels(desc.p, tabs, P.n_level, keys.p, leaf_tri[slot].p, boxes.p, items[0].p, items[1].p,
-                      nitems[0].p, nitems[1].p, nodes[slot].p, leaf_parent[slot].p, root[slot].p, leaf0[slot].p, s);
+  const LvFrame f{keys.p, leaf_tri[slot].p, boxes.p, {items[0].p, items[1].p}, {nitems[0].p, nitems[1].p},
+                  nodes[slot].p, leaf_parent[slot].p, root[slot].p, leaf0[slot].p};
+  launch_build_levels(desc.p, tabs, P.n_level, &f, 1, s);
 }
 
 void BatchBodies::export_tree(uint32_t j, int slot, float* verts3, uint64_t* okeys, uint32_t* order,

@@ -19,6 +19,19 @@ constexpr int MAX_CAPS = 64;
 constexpr int REFIT_CHUNK = 2048;
 constexpr int MAX_SLOTS = 64;
 constexpr int MAX_LANES = 4;
+// A rebuilt body can build up to MAX_FRAMES frames in one chain of launches
+// (grid.y = frame): each kernel of the chain takes one pointer per frame.
+constexpr int MAX_FRAMES = 4;
+template <class T>
+struct PF {
+  T* p[MAX_FRAMES];
+};
+template <class T>
+inline PF<T> pf1(T* p) {
+  PF<T> r{};
+  r.p[0] = p;
+  return r;
+}
 
 // Per-frame motion of a capsule-union body (see cc_body_desc), by value.
 struct MotionParams {
@@ -32,6 +45,9 @@ struct MotionParams {
 // factor of capsule k.
 struct MotionFrame {
   float v[12 + MAX_CAPS];
+};
+struct MotionFrames {
+  MotionFrame f[MAX_FRAMES];
 };
 
 // Motion of frame `frame` (t = frame * dt), computed on the host in double
@@ -76,10 +92,21 @@ constexpr uint32_t LV_MAX_LEAVES = 1u << 21;   // leaf indices of the bottom-up 
 std::vector<std::vector<LvBlock>> build_level_tables(const std::vector<BodyDesc>& h,
                                                      const std::vector<uint32_t>& bodies);
 size_t level_chunks0(uint32_t m);
+// The buffers of one frame of a level build.
+struct LvFrame {
+  const uint64_t* keys;
+  const uint4* leaf_tri;
+  const float* boxes;
+  BItem* items[2];
+  uint32_t* nitems[2];
+  Node* nodes;
+  int32_t* leaf_parent;
+  float* root;
+  float* leaf0;
+};
+// nframes (1..MAX_FRAMES) frames of the same tables in each launch (grid.y)
 void launch_build_levels(const BodyDesc* desc, const LvBlock* const* level, const uint32_t* n_level,
-                         const uint64_t* keys, const uint4* leaf_tri, const float* boxes, BItem* items0,
-                         BItem* items1, uint32_t* nitems0, uint32_t* nitems1, Node* nodes, int32_t* leaf_parent,
-                         float* root, float* leaf0, hipStream_t s);
+                         const LvFrame* frames, int nframes, hipStream_t s);
 
 // One frame of the body as the collision query reads it.
 struct Slot {
@@ -173,8 +200,11 @@ struct Body {
                                     // entry), its link}, entry ~0 for the slots of folded nodes
 
   int nslots = 1, nlanes = 1;
+  // frames per chain of build launches (rebuilt bodies built bottom-up):
+  // frame q of a chain on lane l uses the scratch of lane l + nlanes * q
+  int chain = 1;
   Slot slot[MAX_SLOTS];
-  Lane lane[MAX_LANES];
+  Lane lane[MAX_LANES * MAX_FRAMES];
 
   bool moving() const { return mode == CC_BODY_REFIT || mode == CC_BODY_REBUILD; }
   // slot holding the pose of frame g
@@ -197,14 +227,22 @@ struct Body {
   }
 
   // set-up: host arrays, vertices stride 3 (already extended); a moving
-  // body gets `slots` frame slots and `lanes` update lanes
+  // body gets `slots` frame slots and `lanes` update lanes, a rebuilt one
+  // built bottom-up `chain` frames per chain of launches
   void init(const float* verts3, uint32_t nv, const uint32_t* tris, uint32_t nf, uint32_t mode, float depth,
-            const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s);
+            const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s,
+            int chain = 1);
 
   // launches, asynchronous on s
   void deform(int k, uint64_t frame, hipStream_t s);   // rest -> slot k pose of `frame`
   void build(int k, int l, hipStream_t s);      // full LBVH build of slot k on lane l (includes the refit)
-  void build_levels(int k, int l, hipStream_t s);   // bottom-up tree + boxes from lane l's keys and boxes
+  // F frames in one chain of launches (grid.y = frame): poses of frames[q]
+  // into slots ks[q]; trees of slots ks[q] with the scratch of lanes ls[q]
+  // (F > 1 only for a body built bottom-up)
+  void deform_frames(int F, const int* ks, const uint64_t* frames, hipStream_t s);
+  void build_frames(int F, const int* ks, const int* ls, hipStream_t s);
+  void update_frames(int F, const int* ks, const int* ls, const uint64_t* frames, hipStream_t s);
+  void build_levels(int F, const int* ks, const int* ls, hipStream_t s);   // bottom-up trees + boxes
   void prim_boxes_of(int k, int l, hipStream_t s);  // lane l's expanded boxes of slot k's pose
   void top_setup(int k, int l, hipStream_t s);  // refit description of slot k's topology into lane l
   // boxes of slot k: the four-wide form's only, when the body is walked

@@ -6,6 +6,14 @@
 #include "cc_body.h"
 
 namespace cc {
+// the pointer of this workgroup's frame (grid.y)
+template <class T>
+__device__ inline T* pick(const PF<T>& a) {
+  return a.p[blockIdx.y];
+}
+}  // namespace cc
+
+namespace cc {
 
 __device__ inline void put6(float* d, const float* mn, const float* mx) {
   d[0] = mn[0]; d[1] = mn[1]; d[2] = mn[2];

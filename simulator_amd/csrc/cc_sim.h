@@ -395,9 +395,19 @@ struct Sim {
         lane_go[l] = go_frame;
         lane_go_valid[l] = true;
         if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
-        for (int j = l; j < G; j += body.nlanes) {
-          const uint64_t f = frame + (uint64_t)(G + j);
-          body.update(body.slot_of(f), l, f, ls);
+        // the lane's frames, body.chain at a time (a rebuilt body builds
+        // them in one chain of launches, frame q with lane l + nlanes * q's
+        // scratch)
+        for (int j0 = l; j0 < G; j0 += body.nlanes * body.chain) {
+          int ks[MAX_FRAMES], lq[MAX_FRAMES];
+          uint64_t fr[MAX_FRAMES];
+          int F = 0;
+          for (int j = j0; j < G && F < body.chain; j += body.nlanes, ++F) {
+            fr[F] = frame + (uint64_t)(G + j);
+            ks[F] = body.slot_of(fr[F]);
+            lq[F] = l + body.nlanes * F;
+          }
+          body.update_frames(F, ks, lq, fr, ls);
         }
         if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
         CC_HIP(hipEventRecord(ev_done[l], ls));

@@ -39,14 +39,16 @@ MotionFrame motion_of(const MotionParams& a, const float* phase, uint64_t frame)
 // Every thread moves one vertex: radial breathing about the closest point of
 // its capsule axis, then R * Y + T; block 0 also stores the frame's motion
 // parameters for the host.
-__global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, float* __restrict__ out,
+__global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, PF<float> out_f,
                                                 uint32_t nv, uint32_t ncaps, const uint32_t* __restrict__ vstart,
                                                 const float* __restrict__ p0, const float* __restrict__ p1,
-                                                MotionFrame mf, float* __restrict__ motion_out) {
+                                                MotionFrames mfs, PF<float> motion_f) {
   __shared__ float sP0[3 * MAX_CAPS], sP1[3 * MAX_CAPS];
   __shared__ uint32_t sV[MAX_CAPS + 1];
   const uint32_t tid = threadIdx.x;
-  const float* sP = mf.v;
+  float* __restrict__ out = pick(out_f);
+  float* __restrict__ motion_out = pick(motion_f);
+  const float* sP = mfs.f[blockIdx.y].v;
   for (uint32_t k = tid; k < 3 * ncaps; k += blockDim.x) {
     sP0[k] = p0[k];
     sP1[k] = p1[k];
@@ -66,11 +68,13 @@ __global__ __launch_bounds__(256) void k_deform(const float* __restrict__ rest, 
 constexpr int BOX_THREADS = 256;
 
 // expanded box per triangle + one partial root box per block
-__global__ __launch_bounds__(BOX_THREADS) void k_prim_boxes(const float* __restrict__ verts,
+__global__ __launch_bounds__(BOX_THREADS) void k_prim_boxes(PF<const float> verts_f,
                                                             const uint32_t* __restrict__ tris, uint32_t m,
-                                                            float depth, float* __restrict__ boxes,
-                                                            float* __restrict__ partial) {
+                                                            float depth, PF<float> boxes_f, PF<float> partial_f) {
   __shared__ float red[6][BOX_THREADS];
+  const float* __restrict__ verts = pick(verts_f);
+  float* __restrict__ boxes = pick(boxes_f);
+  float* __restrict__ partial = pick(partial_f);
   const uint32_t i = blockIdx.x * BOX_THREADS + threadIdx.x;
   float mn[3], mx[3];
   box_sentinel(mn, mx);
@@ -100,11 +104,15 @@ __global__ __launch_bounds__(BOX_THREADS) void k_prim_boxes(const float* __restr
 
 // root box (sentinel U all partials, recomputed per block) then the code of
 // each box centroid (compute_morton_bbox, bvh.cu:71-79)
-__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ boxes, uint32_t m,
-                                                const float* __restrict__ partial, uint32_t npart,
-                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
-                                                float* __restrict__ root_out) {
+__global__ __launch_bounds__(256) void k_morton(PF<const float> boxes_f, uint32_t m, PF<const float> partial_f,
+                                                uint32_t npart, PF<uint32_t> codes_f, PF<uint32_t> idx_f,
+                                                PF<float> root_f) {
   __shared__ float red[6][256];
+  const float* __restrict__ boxes = pick(boxes_f);
+  const float* __restrict__ partial = pick(partial_f);
+  uint32_t* __restrict__ codes = pick(codes_f);
+  uint32_t* __restrict__ idx = pick(idx_f);
+  float* __restrict__ root_out = pick(root_f);
   float mn[3], mx[3];
   box_sentinel(mn, mx);
   for (uint32_t q = threadIdx.x; q < npart; q += 256)
@@ -148,10 +156,11 @@ constexpr int RADIX_DIGITS = 1 << RADIX_BITS;
 constexpr int RADIX_PASSES = 3;
 constexpr int DIGITS_PER_THREAD = RADIX_DIGITS / SORT_THREADS;
 
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                             int shift, uint32_t* __restrict__ hist,
-                                                             uint32_t nblk) {
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(PF<const uint32_t> keys_f, uint32_t n, int shift,
+                                                             PF<uint32_t> hist_f, uint32_t nblk) {
   __shared__ uint32_t h[RADIX_DIGITS];
+  const uint32_t* __restrict__ keys = pick(keys_f);
+  uint32_t* __restrict__ hist = pick(hist_f);
   for (int q = 0; q < DIGITS_PER_THREAD; ++q) h[q * SORT_THREADS + threadIdx.x] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * SORT_TILE;
@@ -168,8 +177,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
 
 // Per digit, the exclusive prefix over blocks of the block counts (in place)
 // and the digit's total: one wave per digit, 64 blocks per round.
-__global__ __launch_bounds__(256) void k_radix_rowscan(uint32_t* __restrict__ hist, uint32_t nblk,
-                                                       uint32_t* __restrict__ total) {
+__global__ __launch_bounds__(256) void k_radix_rowscan(PF<uint32_t> hist_f, uint32_t nblk, PF<uint32_t> total_f) {
+  uint32_t* __restrict__ hist = pick(hist_f);
+  uint32_t* __restrict__ total = pick(total_f);
   const uint32_t lane = threadIdx.x & 63u, d = blockIdx.x * 4 + (threadIdx.x >> 6);
   uint32_t* row = hist + (size_t)d * nblk;
   uint32_t carry = 0;
@@ -190,13 +200,17 @@ __global__ __launch_bounds__(256) void k_radix_rowscan(uint32_t* __restrict__ hi
 // Scatter with stable ranks: within a wave, lanes holding the same digit are
 // matched with RADIX_BITS ballots; waves are ordered through per-digit LDS
 // counts.  Thread t owns digits t, t + 256, t + 512, t + 768 in the scans.
-__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin, uint32_t n,
-                                                                int shift, const uint32_t* __restrict__ hist,
-                                                                const uint32_t* __restrict__ total,
-                                                                uint32_t nblk, uint32_t* __restrict__ kout,
-                                                                uint32_t* __restrict__ vout) {
+__global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(PF<const uint32_t> kin_f, PF<const uint32_t> vin_f,
+                                                                uint32_t n, int shift, PF<const uint32_t> hist_f,
+                                                                PF<const uint32_t> total_f, uint32_t nblk,
+                                                                PF<uint32_t> kout_f, PF<uint32_t> vout_f) {
   __shared__ uint32_t goff[RADIX_DIGITS];
+  const uint32_t* __restrict__ kin = pick(kin_f);
+  const uint32_t* __restrict__ vin = pick(vin_f);
+  const uint32_t* __restrict__ hist = pick(hist_f);
+  const uint32_t* __restrict__ total = pick(total_f);
+  uint32_t* __restrict__ kout = pick(kout_f);
+  uint32_t* __restrict__ vout = pick(vout_f);
   __shared__ uint32_t run[RADIX_DIGITS];
   __shared__ uint32_t wsum[SORT_THREADS / 64];
   __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX_DIGITS];
@@ -260,12 +274,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint32_t* 
   }
 }
 
-__global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict__ codes,
-                                                     const uint32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void k_sort_finish(PF<const uint32_t> codes_f, PF<const uint32_t> idx_f,
                                                      const uint32_t* __restrict__ tris, uint32_t m,
-                                                     uint64_t* __restrict__ keys, uint4* __restrict__ leaf_tri) {
+                                                     PF<uint64_t> keys_f, PF<uint4> leaf_tri_f) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= m) return;
+  const uint32_t* __restrict__ codes = pick(codes_f);
+  const uint32_t* __restrict__ idx = pick(idx_f);
+  uint64_t* __restrict__ keys = pick(keys_f);
+  uint4* __restrict__ leaf_tri = pick(leaf_tri_f);
   const uint32_t p = idx[i];
   keys[i] = ((uint64_t)codes[i] << 32) | p;
   const uint32_t* t = tris + 3 * (size_t)p;
@@ -285,17 +302,22 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ key
 // ------------------------------------------------------------ host side
 void Body::prim_boxes_of(int k, int l, hipStream_t s) {
   Lane& L = lane[l];
-  k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(slot[k].verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
+  k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(pf1<const float>(slot[k].verts.p), tris.p, m, depth,
+                                                  pf1(L.boxes.p), pf1(L.partial.p));
   CC_LAUNCHED();
 }
 
-void Body::build_levels(int k, int l, hipStream_t s) {
-  Lane& L = lane[l];
-  Slot& S = slot[k];
+void Body::build_levels(int F, const int* ks, const int* ls, hipStream_t s) {
   const LvBlock* tabs[BB_MAX_LEVELS];
   for (int q = 0; q < BB_MAX_LEVELS; ++q) tabs[q] = bu_level[q].p;
-  launch_build_levels(bu_desc.p, tabs, bu_n, L.keys.p, leaf_tri_of(k), L.boxes.p, L.items[0].p, L.items[1].p,
-                      L.nitems[0].p, L.nitems[1].p, S.nodes.p, L.leaf_parent.p, S.root.p, S.leaf0_box.p, s);
+  LvFrame fr[MAX_FRAMES];
+  for (int f = 0; f < F; ++f) {
+    Lane& L = lane[ls[f]];
+    Slot& S = slot[ks[f]];
+    fr[f] = LvFrame{L.keys.p, leaf_tri_of(ks[f]), L.boxes.p, {L.items[0].p, L.items[1].p},
+                    {L.nitems[0].p, L.nitems[1].p}, S.nodes.p, L.leaf_parent.p, S.root.p, S.leaf0_box.p};
+  }
+  launch_build_levels(bu_desc.p, tabs, bu_n, fr, F, s);
 }
 
 void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
@@ -345,7 +367,7 @@ void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
 }
 
 void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, uint32_t mode_, float depth_,
-                const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s) {
+                const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s, int chain_) {
   CC_REQUIRE(mode_ <= CC_BODY_REBUILD, "body: unknown mode");
   mode = mode_;
   nv = nv_;
@@ -361,6 +383,8 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   CC_REQUIRE(nslots >= 1 && nslots <= MAX_SLOTS && nlanes >= 1 && nlanes <= MAX_LANES, "body: slot/lane count");
   rest.upload(v3, 3 * (size_t)nv, s);
   tris.upload(t3, 3 * (size_t)m, s);
+  chain = (mode == CC_BODY_REBUILD) ? chain_ : 1;
+  CC_REQUIRE(chain >= 1 && chain <= MAX_FRAMES, "body: frames per build chain must be 1..4");
   if (moving()) {
     CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
     CC_REQUIRE(caps->vstart[0] == 0 && caps->vstart[caps->ncaps] == nv, "body: capsule ranges must cover vertices");
@@ -402,46 +426,84 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
       if (bu_n[q]) bu_level[q].upload(t[q].data(), t[q].size(), s);
     }
   }
-  for (int l = 0; l < nlanes; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, bottom_up, s);
+  if (!bottom_up) chain = 1;
+  for (int l = 0; l < nlanes * chain; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, bottom_up, s);
 }
 
-void Body::deform(int k, uint64_t frame, hipStream_t s) {
+void Body::deform_frames(int F, const int* ks, const uint64_t* frames, hipStream_t s) {
   if (nv == 0 || ncaps == 0) return;
-  const MotionFrame mf = motion_of(motion, h_phase.data(), frame);
-  k_deform<<<grid_for(nv, 256), 256, 0, s>>>(rest.p, slot[k].verts.p, nv, ncaps, cap_vstart.p, cap_p0.p, cap_p1.p,
-                                               mf, slot[k].motion.p);
+  MotionFrames mf;
+  PF<float> out{}, mot{};
+  for (int f = 0; f < F; ++f) {
+    mf.f[f] = motion_of(motion, h_phase.data(), frames[f]);
+    out.p[f] = slot[ks[f]].verts.p;
+    mot.p[f] = slot[ks[f]].motion.p;
+  }
+  k_deform<<<dim3(grid_for(nv, 256), (unsigned)F), 256, 0, s>>>(rest.p, out, nv, ncaps, cap_vstart.p, cap_p0.p,
+                                                               cap_p1.p, mf, mot);
   CC_LAUNCHED();
 }
 
-void Body::build(int k, int l, hipStream_t s) {
+void Body::deform(int k, uint64_t frame, hipStream_t s) { deform_frames(1, &k, &frame, s); }
+
+void Body::build_frames(int F, const int* ks, const int* ls, hipStream_t s) {
   if (mode == CC_BODY_NONE || m == 0) return;
-  Lane& L = lane[l];
-  CC_REQUIRE(L.full, "body: build on a refit-only lane");
-  Slot& S = slot[k];
-  k_prim_boxes<<<L.nblk_box, BOX_THREADS, 0, s>>>(S.verts.p, tris.p, m, depth, L.boxes.p, L.partial.p);
+  CC_REQUIRE(F >= 1 && F <= MAX_FRAMES && (F == 1 || bottom_up), "body: frames per build chain");
+  PF<const float> verts{}, boxes_c{}, partial_c{};
+  PF<float> boxes{}, partial{}, root{};
+  PF<uint32_t> code[2] = {}, idx[2] = {}, hist{}, total{};
+  PF<const uint32_t> code_c[2] = {}, idx_c[2] = {}, hist_c{}, total_c{};
+  PF<uint64_t> keys{};
+  PF<uint4> leaf{};
+  for (int f = 0; f < F; ++f) {
+    Lane& L = lane[ls[f]];
+    CC_REQUIRE(L.full, "body: build on a refit-only lane");
+    Slot& S = slot[ks[f]];
+    verts.p[f] = S.verts.p;
+    boxes.p[f] = L.boxes.p;
+    boxes_c.p[f] = L.boxes.p;
+    partial.p[f] = L.partial.p;
+    partial_c.p[f] = L.partial.p;
+    root.p[f] = S.root.p;
+    for (int b = 0; b < 2; ++b) {
+      code[b].p[f] = L.code[b].p;
+      code_c[b].p[f] = L.code[b].p;
+      idx[b].p[f] = L.idx[b].p;
+      idx_c[b].p[f] = L.idx[b].p;
+    }
+    hist.p[f] = L.hist.p;
+    hist_c.p[f] = L.hist.p;
+    total.p[f] = L.digit_total.p;
+    total_c.p[f] = L.digit_total.p;
+    keys.p[f] = L.keys.p;
+    leaf.p[f] = leaf_tri_of(ks[f]);
+  }
+  const unsigned nf = (unsigned)F;
+  const uint32_t nblk_box = lane[ls[0]].nblk_box;
+  k_prim_boxes<<<dim3(nblk_box, nf), BOX_THREADS, 0, s>>>(verts, tris.p, m, depth, boxes, partial);
   CC_LAUNCHED();
-  k_morton<<<grid_for(m, 256), 256, 0, s>>>(L.boxes.p, m, L.partial.p, L.nblk_box, L.code[0].p, L.idx[0].p,
-                                            S.root.p);
+  k_morton<<<dim3(grid_for(m, 256), nf), 256, 0, s>>>(boxes_c, m, partial_c, nblk_box, code[0], idx[0], root);
   CC_LAUNCHED();
   const uint32_t nblk = grid_for(m, SORT_TILE);
   int src = 0;
   for (int pass = 0, shift = 0; pass < RADIX_PASSES; ++pass, shift += RADIX_BITS) {
-    k_radix_hist<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, m, shift, L.hist.p, nblk);
-    k_radix_rowscan<<<RADIX_DIGITS / 4, 256, 0, s>>>(L.hist.p, nblk, L.digit_total.p);
-    k_radix_scatter<<<nblk, SORT_THREADS, 0, s>>>(L.code[src].p, L.idx[src].p, m, shift, L.hist.p,
-                                                  L.digit_total.p, nblk, L.code[src ^ 1].p, L.idx[src ^ 1].p);
+    k_radix_hist<<<dim3(nblk, nf), SORT_THREADS, 0, s>>>(code_c[src], m, shift, hist, nblk);
+    k_radix_rowscan<<<dim3(RADIX_DIGITS / 4, nf), 256, 0, s>>>(hist, nblk, total);
+    k_radix_scatter<<<dim3(nblk, nf), SORT_THREADS, 0, s>>>(code_c[src], idx_c[src], m, shift, hist_c, total_c,
+                                                            nblk, code[src ^ 1], idx[src ^ 1]);
     CC_LAUNCHED();
     src ^= 1;
   }
-  k_sort_finish<<<grid_for(m, 256), 256, 0, s>>>(L.code[src].p, L.idx[src].p, tris.p, m, L.keys.p,
-                                                 leaf_tri_of(k));
+  k_sort_finish<<<dim3(grid_for(m, 256), nf), 256, 0, s>>>(code_c[src], idx_c[src], tris.p, m, keys, leaf);
   CC_LAUNCHED();
   if (bottom_up) {                      // tree, parent links and boxes, bottom-up
-    build_levels(k, l, s);
+    build_levels(F, ks, ls, s);
     return;
   }
+  const int k = ks[0], l = ls[0];
+  Lane& L = lane[l];
   if (m >= 2) {
-    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(L.keys.p, (int32_t)m, S.nodes.p, L.leaf_parent.p,
+    k_karras<<<grid_for(m - 1, 256), 256, 0, s>>>(L.keys.p, (int32_t)m, slot[k].nodes.p, L.leaf_parent.p,
                                                   L.top_count.p);
     CC_LAUNCHED();
   }
@@ -450,6 +512,8 @@ void Body::build(int k, int l, hipStream_t s) {
   refit(k, l, s);
   wide_setup(k, s);
 }
+
+void Body::build(int k, int l, hipStream_t s) { build_frames(1, &k, &l, s); }
 
 void Body::share_topology(int from, hipStream_t s) {
   if (mode != CC_BODY_REFIT || m == 0) return;
@@ -462,6 +526,16 @@ void Body::share_topology(int from, hipStream_t s) {
     CC_HIP(hipMemcpyAsync(S.leaf0_box.p, F.leaf0_box.p, 6 * sizeof(float), hipMemcpyDeviceToDevice, s));
     if (nwide) CC_HIP(hipMemcpyAsync(S.wide.p, F.wide.p, F.wide.n * sizeof(float4), hipMemcpyDeviceToDevice, s));
   }
+}
+
+void Body::update_frames(int F, const int* ks, const int* ls, const uint64_t* frames, hipStream_t s) {
+  if (F == 1) {
+    update(ks[0], ls[0], frames[0], s);
+    return;
+  }
+  CC_REQUIRE(mode == CC_BODY_REBUILD && bottom_up && F <= chain, "body: frames per build chain");
+  deform_frames(F, ks, frames, s);
+  build_frames(F, ks, ls, s);
 }
 
 void Body::update(int k, int l, uint64_t frame, hipStream_t s) {

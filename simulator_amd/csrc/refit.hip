@@ -544,7 +544,7 @@ void Body::refit(int k, int l, hipStream_t s, bool binary) {
   if (mode == CC_BODY_NONE || m == 0) return;
   if (bottom_up) {                      // same keys, so the same tree; new boxes
     prim_boxes_of(k, l, s);
-    build_levels(k, l, s);
+    build_levels(1, &k, &l, s);
     return;
   }
   Lane& L = lane[l];

@@ -417,8 +417,13 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       if (const char* e = std::getenv("CC_GRAPH_STEPS")) S.graph_steps = std::atoi(e);
       CC_REQUIRE(S.graph_steps >= 2 && S.graph_steps % 2 == 0 && 2 * S.graph_steps <= MAX_SLOTS,
                  "cc_sim_create: CC_GRAPH_STEPS must be even, 2..MAX_SLOTS/2");
+      // a rebuilt body's lanes build CHAIN frames per chain of launches in
+      // the step graphs' blocks
+      int chain = 2;
+      if (const char* e = std::getenv("CC_BUILD_FRAMES")) chain = std::atoi(e);
+      CC_REQUIRE(chain >= 1 && chain <= MAX_FRAMES, "cc_sim_create: CC_BUILD_FRAMES must be 1..4");
       S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, mp,
-                  2 * S.graph_steps, lanes, s);
+                  2 * S.graph_steps, lanes, s, chain);
       // frame 0 into slot 0; a refitted body keeps the topology built from
       // the frame-0 pose in every slot
       S.body.deform(0, 0, s);
