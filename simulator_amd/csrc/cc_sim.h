@@ -2,6 +2,7 @@
 // collider body, the step graphs and the update lanes.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "cc_body.h"
@@ -116,6 +117,9 @@ struct Sim {
   static constexpr int TRING = 256;
   hipEvent_t rev[TRING][2] = {};
   uint8_t rkind[TRING] = {};     // 0: fused step, 1: body update
+  uint8_t rcount[TRING] = {};    // frames the bracketed launches cover
+  uint32_t tcount[2][MAX_LANES] = {};   // frames of each lane in a timed block
+  uint64_t chains_issued = 0;
   uint64_t r_head = 0, r_tail = 0;
 
   // Work still queued on the streams (a creation that failed part way, lane
@@ -245,7 +249,7 @@ struct Sim {
         CC_HIP(hipEventSynchronize(te[3 + 2 * l]));
         CC_HIP(hipEventElapsedTime(&ms, te[2 + 2 * l], te[3 + 2 * l]));
         t_update_ms += ms;
-        t_updates += (uint64_t)((G - l + body.nlanes - 1) / body.nlanes);
+        t_updates += tcount[set][l];
       }
     tpending[set] = false;
   }
@@ -273,7 +277,7 @@ struct Sim {
       ++t_steps;
     } else {
       t_update_ms += ms;
-      ++t_updates;
+      t_updates += rcount[e];
     }
     ++r_tail;
   }
@@ -281,10 +285,11 @@ struct Sim {
     while (r_tail < r_head) ring_collect_one();
   }
   // an event pair of the ring (the older pair in its place is read first)
-  hipEvent_t* ring_next(uint8_t kind) {
+  hipEvent_t* ring_next(uint8_t kind, uint8_t count = 1) {
     if (r_head - r_tail == (uint64_t)TRING) ring_collect_one();
     const int e = (int)(r_head % TRING);
     rkind[e] = kind;
+    rcount[e] = count;
     ++r_head;
     return rev[e];
   }
@@ -292,8 +297,15 @@ struct Sim {
   // Lane work of the single-step path: the poses of frames [ready, upto]
   // on lanes g % nlanes, each after a go that covers its slot (see
   // GO_EVERY; the caller has recorded one at most GO_EVERY frames ago, and
-  // upto + GO_EVERY < frame + nslots).
+  // upto + GO_EVERY < frame + nslots).  A body built `chain` frames per
+  // chain of launches takes whole groups of nlanes * chain frames from
+  // `ready` instead (lane l: frames ready + l + nlanes * q), so up to
+  // nlanes * chain - 1 frames past upto.
   void issue_ahead(uint64_t upto) {
+    if (body.chain > 1) {
+      issue_chains(upto);
+      return;
+    }
     for (; ready <= upto; ++ready) {
       const uint64_t g = ready;
       const int l = (int)(g % (uint64_t)body.nlanes);
@@ -315,6 +327,42 @@ struct Sim {
       lane_pending[l] = true;
       pose_pending[k] = true;
       lanes_inflight = true;
+    }
+  }
+
+  void issue_chains(uint64_t upto) {
+    const int NL = body.nlanes, C = body.chain;
+    while (ready <= upto) {
+      const uint64_t base = ready;
+      for (int l = 0; l < NL; ++l) {
+        int ks[MAX_FRAMES], lq[MAX_FRAMES];
+        uint64_t fr[MAX_FRAMES];
+        for (int q = 0; q < C; ++q) {
+          fr[q] = base + (uint64_t)(l + NL * q);
+          ks[q] = body.slot_of(fr[q]);
+          lq[q] = l + NL * q;
+        }
+        hipStream_t ls = lane_s[l];
+        if (!lane_go_valid[l] || lane_go[l] + (uint64_t)body.nslots <= fr[C - 1]) {
+          CC_HIP(hipStreamWaitEvent(ls, ev_go, 0));
+          lane_go[l] = go_frame;
+          lane_go_valid[l] = true;
+        }
+        // one chain in four timed (see issue_ahead)
+        hipEvent_t* te = (timing && chains_issued % 4 == 0) ? ring_next(1, (uint8_t)C) : nullptr;
+        ++chains_issued;
+        if (te) CC_HIP(hipEventRecord(te[0], ls));
+        body.update_frames(C, ks, lq, fr, ls);
+        if (te) CC_HIP(hipEventRecord(te[1], ls));
+        for (int q = 0; q < C; ++q) {
+          CC_HIP(hipEventRecord(ev_pose[ks[q]], ls));
+          pose_pending[ks[q]] = true;
+        }
+        lane_last[l] = ev_pose[ks[C - 1]];
+        lane_pending[l] = true;
+        lanes_inflight = true;
+      }
+      ready = base + (uint64_t)(NL * C);
     }
   }
 
@@ -395,20 +443,25 @@ struct Sim {
         lane_go[l] = go_frame;
         lane_go_valid[l] = true;
         if (te) CC_HIP(hipEventRecord(te[2 + 2 * l], ls));
-        // the lane's frames, body.chain at a time (a rebuilt body builds
-        // them in one chain of launches, frame q with lane l + nlanes * q's
-        // scratch)
-        for (int j0 = l; j0 < G; j0 += body.nlanes * body.chain) {
+        // the lane's frames of the next block not issued yet (single steps
+        // of a chained body may have issued its first ones), body.chain at
+        // a time (a rebuilt body builds them in one chain of launches, frame
+        // q with lane l + nlanes * q's scratch)
+        const uint64_t first = std::max(ready, frame + (uint64_t)G), end = frame + 2 * (uint64_t)G;
+        uint32_t nf = 0;
+        for (uint64_t f0 = first + (uint64_t)l; f0 < end; f0 += (uint64_t)(body.nlanes * body.chain)) {
           int ks[MAX_FRAMES], lq[MAX_FRAMES];
           uint64_t fr[MAX_FRAMES];
           int F = 0;
-          for (int j = j0; j < G && F < body.chain; j += body.nlanes, ++F) {
-            fr[F] = frame + (uint64_t)(G + j);
-            ks[F] = body.slot_of(fr[F]);
+          for (uint64_t f = f0; f < end && F < body.chain; f += (uint64_t)body.nlanes, ++F) {
+            fr[F] = f;
+            ks[F] = body.slot_of(f);
             lq[F] = l + body.nlanes * F;
           }
           body.update_frames(F, ks, lq, fr, ls);
+          nf += (uint32_t)F;
         }
+        if (te) tcount[tset][l] = nf;
         if (te) CC_HIP(hipEventRecord(te[3 + 2 * l], ls));
         CC_HIP(hipEventRecord(ev_done[l], ls));
         lane_last[l] = ev_done[l];

@@ -443,7 +443,10 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
         // single steps keep two updates per lane in flight
         S.lookahead = 2 * S.body.nlanes;
         if (const char* e = std::getenv("CC_LOOKAHEAD")) S.lookahead = std::atoi(e);
-        CC_REQUIRE(S.lookahead >= 1 && S.lookahead + Sim::GO_EVERY + 1 < S.body.nslots,
+        // single steps of a chained body issue up to nlanes * chain - 1
+        // frames past the lookahead
+        CC_REQUIRE(S.lookahead >= 1 &&
+                       S.lookahead + S.body.nlanes * S.body.chain + Sim::GO_EVERY < S.body.nslots,
                    "cc_sim_create: CC_LOOKAHEAD too large for the slot ring");
       }
     }
