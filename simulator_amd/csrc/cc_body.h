@@ -248,6 +248,9 @@ struct Body {
   // boxes of slot k: the four-wide form's only, when the body is walked
   // through it, unless `binary`
   void refit(int k, int l, hipStream_t s, bool binary = false);
+  // F frames' refits in one chain of launches (grid.y = frame; a refitted
+  // body: every slot shares lane 0's topology)
+  void refit_frames(int F, const int* ks, const int* ls, hipStream_t s, bool binary = false);
   void update(int k, int l, uint64_t frame, hipStream_t s);   // per-mode update of slot k to `frame`
   void share_topology(int from, hipStream_t s); // copy slot `from`'s tree to every other slot
   // Four-wide form: the tables from slot k's tree and slot k's wide records

@@ -383,7 +383,7 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   CC_REQUIRE(nslots >= 1 && nslots <= MAX_SLOTS && nlanes >= 1 && nlanes <= MAX_LANES, "body: slot/lane count");
   rest.upload(v3, 3 * (size_t)nv, s);
   tris.upload(t3, 3 * (size_t)m, s);
-  chain = (mode == CC_BODY_REBUILD) ? chain_ : 1;
+  chain = moving() ? chain_ : 1;
   CC_REQUIRE(chain >= 1 && chain <= MAX_FRAMES, "body: frames per build chain must be 1..4");
   if (moving()) {
     CC_REQUIRE(caps && caps->ncaps >= 1 && caps->ncaps <= MAX_CAPS, "body: capsule layout required (1..64)");
@@ -426,7 +426,7 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
       if (bu_n[q]) bu_level[q].upload(t[q].data(), t[q].size(), s);
     }
   }
-  if (!bottom_up) chain = 1;
+  if (mode == CC_BODY_REBUILD && !bottom_up) chain = 1;
   for (int l = 0; l < nlanes * chain; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, bottom_up, s);
 }
 
@@ -533,9 +533,12 @@ void Body::update_frames(int F, const int* ks, const int* ls, const uint64_t* fr
     update(ks[0], ls[0], frames[0], s);
     return;
   }
-  CC_REQUIRE(mode == CC_BODY_REBUILD && bottom_up && F <= chain, "body: frames per build chain");
+  CC_REQUIRE(F <= chain, "body: frames per chain");
   deform_frames(F, ks, frames, s);
-  build_frames(F, ks, ls, s);
+  if (mode == CC_BODY_REFIT)
+    refit_frames(F, ks, ls, s);
+  else
+    build_frames(F, ks, ls, s);
 }
 
 void Body::update(int k, int l, uint64_t frame, hipStream_t s) {

@@ -215,11 +215,17 @@ __global__ __launch_bounds__(256) void k_node_links(const Node* __restrict__ nod
 
 // ======================================================== per-frame refit
 __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
-    const float* __restrict__ verts, const uint4* __restrict__ leaf_tri, const int32_t* __restrict__ leaf_parent,
-    const int4* __restrict__ links, Node* __restrict__ nodes, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
+    PF<const float> verts_f, const uint4* __restrict__ leaf_tri, const int32_t* __restrict__ leaf_parent,
+    const int4* __restrict__ links, PF<Node> nodes_f, uint32_t m, float depth, const uint8_t* __restrict__ nflag,
     const uint8_t* __restrict__ lflag, const uint32_t* __restrict__ chunk_h, const int32_t* __restrict__ top_pos,
-    float* __restrict__ top_child, float* __restrict__ root_out, float* __restrict__ leaf0_box,
-    float4* __restrict__ wide, const uint2* __restrict__ wdst) {
+    PF<float> top_child_f, PF<float> root_f, PF<float> leaf0_f, PF<float4> wide_f,
+    const uint2* __restrict__ wdst) {
+  const float* __restrict__ verts = pick(verts_f);
+  Node* __restrict__ nodes = pick(nodes_f);
+  float* __restrict__ top_child = pick(top_child_f);
+  float* __restrict__ root_out = pick(root_f);
+  float* __restrict__ leaf0_box = pick(leaf0_f);
+  float4* __restrict__ wide = pick(wide_f);
   __shared__ float lbox[RC_CHUNK][6];
   __shared__ float nbox[RC_CHUNK][6];
   const int32_t c0 = (int32_t)(blockIdx.x * RC_CHUNK);
@@ -487,15 +493,19 @@ __device__ void refit_top_levels(Node* __restrict__ nodes, uint32_t T, uint32_t 
   }
 }
 
-__global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(Node* __restrict__ nodes,
-                                                                 const int4* __restrict__ links,
+__global__ __launch_bounds__(TOP_REFIT_THREADS) void k_refit_top(PF<Node> nodes_f, const int4* __restrict__ links,
                                                                  const uint32_t* __restrict__ lev,
-                                                                 const float* __restrict__ top_child,
+                                                                 PF<const float> top_child_f,
                                                                  const uint32_t* __restrict__ top_count,
                                                                  const uint32_t* __restrict__ top_hmax,
-                                                                 float* root_out, float* __restrict__ scratch,
-                                                                 float4* __restrict__ wide,
+                                                                 PF<float> root_f, PF<float> scratch_f,
+                                                                 PF<float4> wide_f,
                                                                  const uint2* __restrict__ wdst) {
+  Node* __restrict__ nodes = pick(nodes_f);
+  const float* __restrict__ top_child = pick(top_child_f);
+  float* root_out = pick(root_f);
+  float* __restrict__ scratch = pick(scratch_f);
+  float4* __restrict__ wide = pick(wide_f);
   __shared__ float sbox[TOP_MAX_LDS * 6];
   __shared__ int2 slnk[TOP_MAX_LDS];
   __shared__ uint32_t slev[TOP_MAX_LEVELS + 2];
@@ -547,22 +557,42 @@ void Body::refit(int k, int l, hipStream_t s, bool binary) {
     build_levels(1, &k, &l, s);
     return;
   }
-  Lane& L = lane[l];
-  Lane& T = topo(l);
-  Slot& S = slot[k];
+  refit_frames(1, &k, &l, s, binary);
+}
+
+void Body::refit_frames(int F, const int* ks, const int* ls, hipStream_t s, bool binary) {
+  CC_REQUIRE(F >= 1 && F <= MAX_FRAMES && !bottom_up, "body: frames per refit chain");
+  // every frame of a chain shares the topology (that of lane 0 for a
+  // refitted body, of lane ls[0] for a rebuilt one refitted in place)
+  Lane& T = topo(ls[0]);
   // A body walked through its four-wide form gets only the wide boxes (the
   // refit reads just the links of the binary records); `binary` asks for
   // the binary boxes instead (export, or the binary walk).
-  float4* W = (nwide && wide_enabled && !binary) ? S.wide.p : nullptr;
-  k_refit_chunks<<<grid_for(m, RC_CHUNK), RC_THREADS, 0, s>>>(S.verts.p, leaf_tri_of(k), T.leaf_parent.p,
-                                                              T.links.p, S.nodes.p, m, depth, T.nflag.p, T.lflag.p,
-                                                              T.chunk_h.p, T.top_pos.p, L.top_child.p, S.root.p,
-                                                              S.leaf0_box.p, W, w_dst.p);
+  const bool W = nwide && wide_enabled && !binary;
+  PF<const float> verts{}, top_child_c{};
+  PF<Node> nodes{};
+  PF<float> top_child{}, root{}, leaf0{}, scratch{};
+  PF<float4> wide{};
+  for (int f = 0; f < F; ++f) {
+    Lane& L = lane[ls[f]];
+    Slot& S = slot[ks[f]];
+    verts.p[f] = S.verts.p;
+    nodes.p[f] = S.nodes.p;
+    top_child.p[f] = L.top_child.p;
+    top_child_c.p[f] = L.top_child.p;
+    root.p[f] = S.root.p;
+    leaf0.p[f] = S.leaf0_box.p;
+    scratch.p[f] = L.top_scratch.p;
+    wide.p[f] = W ? S.wide.p : nullptr;
+  }
+  k_refit_chunks<<<dim3(grid_for(m, RC_CHUNK), (unsigned)F), RC_THREADS, 0, s>>>(
+      verts, leaf_tri_of(ks[0]), T.leaf_parent.p, T.links.p, nodes, m, depth, T.nflag.p, T.lflag.p, T.chunk_h.p,
+      T.top_pos.p, top_child, root, leaf0, wide, w_dst.p);
   CC_LAUNCHED();
   if (m > (uint32_t)RC_CHUNK) {
-    k_refit_top<<<1, TOP_REFIT_THREADS, 0, s>>>(S.nodes.p, T.top_links.p, T.top_lev.p, L.top_child.p,
-                                                T.top_count.p, T.top_hmax.p, S.root.p, L.top_scratch.p, W,
-                                                w_dst.p);
+    k_refit_top<<<dim3(1, (unsigned)F), TOP_REFIT_THREADS, 0, s>>>(nodes, T.top_links.p, T.top_lev.p, top_child_c,
+                                                                   T.top_count.p, T.top_hmax.p, root, scratch, wide,
+                                                                   w_dst.p);
     CC_LAUNCHED();
   }
 }
