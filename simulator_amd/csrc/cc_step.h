@@ -149,7 +149,11 @@ __device__ inline void step_vertex(uint32_t i, const float* __restrict__ sin_, f
   float* o = sout + 6 * (size_t)i;
   o[0] = np.x; o[1] = np.y; o[2] = np.z;
   o[3] = nq.x; o[4] = nq.y; o[5] = nq.z;
-  st3(contact_out + 3 * (size_t)i, nc);
+  // an update in place stores the normal only where it changed (bit for
+  // bit): most vertices keep theirs (0 away from the body) from step to step
+  if (contact_out != contact_in || __float_as_uint(nc.x) != __float_as_uint(c.x) ||
+      __float_as_uint(nc.y) != __float_as_uint(c.y) || __float_as_uint(nc.z) != __float_as_uint(c.z))
+    st3(contact_out + 3 * (size_t)i, nc);
 }
 
 }  // namespace cc
