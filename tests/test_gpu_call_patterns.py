@@ -4,7 +4,7 @@ runs that end inside a block), interleaved with synchronisation, state
 reads, collision queries (allocating and allocation-free), timing switched
 on and off, and the frame set back, against a simulator stepped once per
 frame through the same frame changes.  Moving bodies, refitted and rebuilt:
-the update lanes run up to six frames (single steps) or a block (graphs)
+the update lanes run up to a dozen frames (single steps) or a block (graphs)
 ahead of the step, and every step must still see the pose of its own frame."""
 import numpy as np
 import pytest
