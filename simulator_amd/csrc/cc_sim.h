@@ -84,10 +84,12 @@ struct Sim {
   double t_verlet_ms = 0, t_update_ms = 0;
   uint64_t t_steps = 0, t_updates = 0;
   // Single steps (the reference's one call per frame): the pose of frame g
-  // is computed on lane g % nlanes up to `lookahead` frames ahead of the
-  // step, and ev_pose[slot of g] marks it done; pose_pending says that the
-  // step of the frame in that slot must wait for it.  block_inflight: the
-  // lanes of a step graph block have not been joined yet.
+  // is computed on a lane up to `lookahead` frames ahead of the step (a
+  // body updated one frame per chain: lane g % nlanes; `chain` frames per
+  // chain: in groups, see issue_chains), and ev_pose[slot of g] marks it
+  // done; pose_pending says that the step of the frame in that slot must
+  // wait for it.  block_inflight: the lanes of a step graph block have not
+  // been joined yet.
   int lookahead = 0;
   hipEvent_t ev_pose[MAX_SLOTS] = {};
   bool pose_pending[MAX_SLOTS] = {};
@@ -119,7 +121,7 @@ struct Sim {
   uint8_t rkind[TRING] = {};     // 0: fused step, 1: body update
   uint8_t rcount[TRING] = {};    // frames the bracketed launches cover
   uint32_t tcount[2][MAX_LANES] = {};   // frames of each lane in a timed block
-  uint64_t chains_issued = 0;
+  uint64_t chains_issued = 0;    // single-step chains so far (one in four is timed)
   uint64_t r_head = 0, r_tail = 0;
 
   // Work still queued on the streams (a creation that failed part way, lane
