@@ -319,14 +319,16 @@ int cc_bvh_rebuild(cc_sim* sim);
 int cc_bvh_refit(cc_sim* sim);
 int cc_bvh_size(cc_sim* sim, uint32_t* leaves);
 
-/* Traversal form of a static or refitted body (no counterpart in the
-   reference, whose D_BVH::intersect, cudaBRT.cu:239-285, walks the binary
-   tree).  Width 4 (the default) walks a four-wide form refolded from the
-   binary tree after every refit, entries in the binary walk's order of
+/* Traversal form of the body (no counterpart in the reference, whose
+   D_BVH::intersect, cudaBRT.cu:239-285, walks the binary tree).  Width 4
+   (the default) walks a four-wide form folded from the binary tree after
+   every refit or rebuild, entries in the binary walk's order of
    consideration, so first hits and contacts are the same; width 2 walks the
-   binary tree.  Rebuilt bodies, and trees whose root paths could defer more
-   than 64 entries, always walk the binary tree.  cc_bvh_width reports the
-   form in use, the wide node count and the deferral bound of the tree. */
+   binary tree.  A static or refitted body whose root paths could defer more
+   than 64 entries always walks the binary tree.  cc_bvh_width reports the
+   form in use, the wide node count and the deferral bound of the tree (64,
+   the stack limit, for a rebuilt body: its wide walk defers no more than
+   the binary walk, whose stack holds at most the tree depth). */
 int cc_sim_set_bvh_width(cc_sim* sim, int width);
 int cc_bvh_width(cc_sim* sim, int* width, uint32_t* wide_nodes, uint32_t* stack_bound);
 

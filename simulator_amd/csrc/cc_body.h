@@ -116,8 +116,8 @@ struct Slot {
   DBuf<float> root;        // [6] root box (sentinel U all)
   DBuf<float> leaf0_box;   // [6] box of leaf 0 when m == 1
   DBuf<float> motion;      // [12 + ncaps] motion parameters of the frame
-  DBuf<float4> wide;       // [8 * nwide] four-wide nodes (static and refitted bodies): four
-                           // 32-byte entries {min xyz, max x | max yz, link, 0}
+  DBuf<float4> wide;       // [8 * nwide] four-wide nodes: four 32-byte entries
+                           // {min xyz, max x | max yz, link, split}
 };
 
 // Scratch of one update lane.  A full lane can build a tree; a refit-only
@@ -191,9 +191,12 @@ struct Body {
   // Four-wide form of a static or refitted body's tree (fixed topology):
   // per wide node, the binary records holding its entries' boxes (record *
   // 2 + side, ~0 for an empty entry) and its links (wide index >= 0, ~leaf).
+  // A rebuilt body (`grouped`) has one wide record per internal node,
+  // folded on the device after every build (fold_frames), and no tables.
   uint32_t nwide = 0;
   uint32_t wide_stack = 0;          // most entries a traversal can defer
   bool wide_enabled = true;
+  bool grouped = false;
   DBuf<uint4> w_src;
   DBuf<int4> w_link;
   DBuf<uint2> w_dst;                // [2 * (m - 1)] binary child slot -> {wide entry (record * 4 +
@@ -223,6 +226,7 @@ struct Body {
     v.leaf0_box = slot[k].leaf0_box.p;
     v.m = (mode == CC_BODY_NONE) ? 0 : m;
     v.wide = (nwide && wide_enabled) ? slot[k].wide.p : nullptr;
+    v.grouped = grouped ? 1u : 0u;
     return v;
   }
 
@@ -257,6 +261,9 @@ struct Body {
   // (host, synchronous; keeps the binary form when the stack bound fails);
   // later refits write the boxes through w_dst.
   void wide_setup(int k, hipStream_t s);
+  // Rebuilt bodies: the wide records of slots ks[q] from their binary trees
+  // (grid.y = frame; nothing when the binary walk is selected).
+  void fold_frames(int F, const int* ks, hipStream_t s);
 
   // host export in the reference encoding (lane 0 holds the keys and leaf
   // parents of slot k's tree)

@@ -168,10 +168,20 @@ __device__ inline WideRoot load_wide_root(const BvhView& b) {
 // returned when popped.  The boxes of the folded nodes are not needed: a
 // node's box is the union of its children's, so it holds p whenever a box
 // below it does.  Entries past the count have an empty box (+inf, -inf).
-// A record is four 32-byte entries {min xyz, max x | max yz, link, 0}.
-// The set-up keeps the wide form only when no root path defers more than
-// 64 entries (sum of entries - 1 over the path), so the stack cannot
-// overflow.
+// A record is four 32-byte entries {min xyz, max x | max yz, link, split}.
+// Two layouts share the walk:
+//  - static and refitted bodies (set up once per topology, on the host):
+//    folded records, split 0; the set-up keeps the wide form only when no
+//    root path defers more than 64 entries (sum of entries - 1 over the
+//    path), so the stack cannot overflow;
+//  - rebuilt bodies (folded on the device every frame): one record per
+//    internal node X at X's index, holding X's leaf children and the
+//    children of its internal children.  When both children A and B are
+//    internal, entries 0-1 are A's children and 2-3 B's, and entry 0's
+//    split word is B's index: while the first candidate lies in A's pair,
+//    the walk defers B's record instead of B's entries.  Then every visit
+//    defers at most one node per binary level it spans, as the binary walk
+//    does, so the stack stays within the tree depth (below 64).
 __device__ inline void wstk_put(const Spill& sp, int k, int32_t v) {
   if (k < WSTK_LDS)
     sp.lds[k * WSTK_LANES] = v;
@@ -199,15 +209,26 @@ __device__ inline int32_t wstk_get(const Spill& sp, int k) {
 // a register stack entry by entry, which matters because all four waves of
 // a SIMD walk at once.
 enum : int { WIDE_NEXT = -1, WIDE_MISS = -2 };
+template <bool GROUPED>
 __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, float4 q3, float4 q4, float4 q5,
                                      float4 q6, float4 q7, const Spill& spill, int& sp, int32_t& node,
                                      int limit) {
   // Kept as booleans (lane masks): the first candidate and the deferred set
   // follow from them by selects, with no per-lane bit mask to build.
-  const bool i0 = in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y), i1 = in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y),
-             i2 = in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y), i3 = in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y);
-  const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c2 = __float_as_int(q5.z),
-                c3 = __float_as_int(q7.z);
+  const bool i0 = in_box(p, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y), i1 = in_box(p, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y);
+  bool i2 = in_box(p, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y), i3 = in_box(p, q6.x, q6.y, q6.z, q6.w, q7.x, q7.y);
+  const int32_t c0 = __float_as_int(q1.z), c1 = __float_as_int(q3.z), c3 = __float_as_int(q7.z);
+  int32_t c2 = __float_as_int(q5.z);
+  if constexpr (GROUPED) {
+    // A split record whose first candidate is in entries 0-1: entries 2-3
+    // become one candidate, the record of their parent (the split word;
+    // the root is no node's child, so a split is never 0).
+    const int32_t split = __float_as_int(q1.w);
+    const bool in_a = (split != 0) & (i0 | i1);
+    c2 = in_a ? split : c2;
+    i2 = in_a ? (i2 | i3) : i2;
+    i3 = i3 & !in_a;
+  }
   if (!(i0 | i1 | i2 | i3)) {
     if (sp == 0) return WIDE_MISS;
     --sp;
@@ -223,8 +244,8 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
   const int32_t x0 = r3 ? c3 : (r2 ? c2 : c1);
   const int32_t x1 = r3 ? (r2 ? c2 : c1) : c1;
   const int k = (int)r1 + (int)r2 + (int)r3;
-  if (sp + k > limit) {                 // unreachable at the default limit: the set-up bounds
-    trav_fault(spill, TRAV_OVERFLOW);   // the deferrals by 64
+  if (sp + k > limit) {                 // unreachable at the default limit (both layouts
+    trav_fault(spill, TRAV_OVERFLOW);   // bound the deferrals by 64)
     return WIDE_MISS;
   }
   if (sp + 3 <= WSTK_LDS) {
@@ -242,12 +263,15 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
   return WIDE_NEXT;
 }
 
+// GROUPED: a rebuilt body's records (split words); the walk of static and
+// refitted bodies is compiled without them.
+template <bool GROUPED>
 __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visits, Spill spill, const WideRoot& rt) {
   int sp = 0;
   int32_t node = 0;
   const int limit = stack_limit(spill);
   ++visits;
-  int32_t r = wide_visit(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node,
+  int32_t r = wide_visit<GROUPED>(p, rt.q[0], rt.q[1], rt.q[2], rt.q[3], rt.q[4], rt.q[5], rt.q[6], rt.q[7], spill, sp, node,
                          limit);
   uint32_t guard = 0;
   for (; r == WIDE_NEXT && guard < b.m; ++guard) {
@@ -262,7 +286,7 @@ __device__ inline int32_t wide_first_hit(const BvhView& b, f3 p, uint32_t& visit
     // memory round trip per descending visit.
     asm volatile("" ::"v"(q1.z), "v"(q3.z), "v"(q5.z), "v"(q7.z));
     ++visits;
-    r = wide_visit(p, q0, q1, q2, q3, q4, q5, q6, q7, spill, sp, node, limit);
+    r = wide_visit<GROUPED>(p, q0, q1, q2, q3, q4, q5, q6, q7, spill, sp, node, limit);
   }
   if (r == WIDE_NEXT) trav_fault(spill, TRAV_BOUND);
   return r >= 0 ? r : -1;
@@ -281,7 +305,8 @@ __device__ inline int32_t bvh_first_hit(const BvhView& b, f3 p, uint32_t& visits
                                        const WideRoot& rt) {
   if (b.m == 0) return -1;
   if (b.m == 1) return box_has(b.leaf0_box, b.leaf0_box + 3, p) ? 0 : -1;
-  if (WIDE && b.wide) return wide_first_hit(b, p, visits, spill, rt);
+  if (WIDE && b.wide)
+    return b.grouped ? wide_first_hit<true>(b, p, visits, spill, rt) : wide_first_hit<false>(b, p, visits, spill, rt);
   TravStack st;
   st.spill = spill;
   st.limit = stack_limit(spill);

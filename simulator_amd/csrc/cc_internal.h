@@ -161,6 +161,9 @@ struct BvhView {
   // nodes of up to four entries (four boxes, four links); nullptr when the
   // body is traversed through `nodes`.
   const float4* wide = nullptr;
+  // the wide records are a rebuilt body's (one per internal node, split
+  // records): the walk takes the split words into account
+  uint32_t grouped = 0;
 };
 
 // --------------------------------------------------- FP32 math, glm order

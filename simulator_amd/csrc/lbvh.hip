@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "cc_device.h"
 #include "cc_lbvh.h"
 
 namespace cc {
@@ -428,6 +429,15 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
   }
   if (mode == CC_BODY_REBUILD && !bottom_up) chain = 1;
   for (int l = 0; l < nlanes * chain; ++l) lane[l].alloc(m, l == 0 || mode == CC_BODY_REBUILD, bottom_up, s);
+  // A rebuilt body is walked through wide records folded after each build,
+  // one per internal node; its walk defers no more than the binary walk's
+  // (the tree depth), so the bound reported is the stack limit.
+  grouped = mode == CC_BODY_REBUILD && m >= 2;
+  if (grouped) {
+    nwide = m - 1;
+    wide_stack = STACK_LIMIT;
+    for (int k = 0; k < nslots; ++k) slot[k].wide.alloc(8 * (size_t)nwide);
+  }
 }
 
 void Body::deform_frames(int F, const int* ks, const uint64_t* frames, hipStream_t s) {
@@ -498,6 +508,7 @@ void Body::build_frames(int F, const int* ks, const int* ls, hipStream_t s) {
   CC_LAUNCHED();
   if (bottom_up) {                      // tree, parent links and boxes, bottom-up
     build_levels(F, ks, ls, s);
+    fold_frames(F, ks, s);
     return;
   }
   const int k = ks[0], l = ls[0];
@@ -508,6 +519,10 @@ void Body::build_frames(int F, const int* ks, const int* ls, hipStream_t s) {
     CC_LAUNCHED();
   }
   top_setup(k, l, s);
+  if (grouped) {         // binary boxes, then the wide records
+    refit(k, l, s);
+    return;
+  }
   nwide = 0;             // a new topology: the wide table follows the refit
   refit(k, l, s);
   wide_setup(k, s);

@@ -558,9 +558,10 @@ void Body::refit(int k, int l, hipStream_t s, bool binary) {
   if (bottom_up) {                      // same keys, so the same tree; new boxes
     prim_boxes_of(k, l, s);
     build_levels(1, &k, &l, s);
-    return;
+  } else {
+    refit_frames(1, &k, &l, s, binary);
   }
-  refit_frames(1, &k, &l, s, binary);
+  fold_frames(1, &k, s);                // a rebuilt body's wide records
 }
 
 void Body::refit_frames(int F, const int* ks, const int* ls, hipStream_t s, bool binary) {
@@ -570,8 +571,9 @@ void Body::refit_frames(int F, const int* ks, const int* ls, hipStream_t s, bool
   Lane& T = topo(ls[0]);
   // A body walked through its four-wide form gets only the wide boxes (the
   // refit reads just the links of the binary records); `binary` asks for
-  // the binary boxes instead (export, or the binary walk).
-  const bool W = nwide && wide_enabled && !binary;
+  // the binary boxes instead (export, or the binary walk).  A rebuilt body
+  // refitted in place gets binary boxes, which fold_frames then folds.
+  const bool W = nwide && wide_enabled && !binary && !grouped;
   PF<const float> verts{}, top_child_c{};
   PF<Node> nodes{};
   PF<float> top_child{}, root{}, leaf0{}, scratch{};

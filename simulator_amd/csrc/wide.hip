@@ -1,4 +1,4 @@
-// wide.hip -- the four-wide form of a static or refitted body's LBVH.
+// wide.hip -- the four-wide form of the body's LBVH.
 //
 // The collision query walks the Karras tree one node record per dependent
 // memory round trip, and the slowest lanes of a step (points resting on the
@@ -16,11 +16,16 @@
 // binary boxes (k_fold_wide).  Later refits write each box straight into the
 // wide entry that mirrors its binary child slot (w_dst) and leave the binary
 // boxes alone; export and the binary walk ask for a binary refit.
+//
+// A rebuilt body's topology changes every frame, so its wide form is folded
+// on the device after each build (k_fold_grouped): one record per internal
+// node at the node's own index, no host set-up, no bound to check (the
+// split records keep the walk's deferrals within the tree depth).
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
-#include "cc_body.h"
+#include "cc_lbvh.h"
 
 namespace cc {
 
@@ -48,6 +53,70 @@ __global__ __launch_bounds__(256) void k_fold_wide(const Node* __restrict__ node
     o[2 * e] = make_float4(b[0], b[1], b[2], b[3]);
     o[2 * e + 1] = make_float4(b[4], b[5], __int_as_float(l4[e]), 0.0f);
   }
+}
+
+// Entry e of record o: box b6 (min xyz, max xyz), link, split word.
+__device__ inline void put_entry(float4* o, int e, const float* mn, const float* mx, int32_t link, int32_t split) {
+  o[2 * e] = make_float4(mn[0], mn[1], mn[2], mx[0]);
+  o[2 * e + 1] = make_float4(mx[1], mx[2], __int_as_float(link), __int_as_float(split));
+}
+__device__ inline void put_empty(float4* o, int e) {
+  o[2 * e] = make_float4(INFINITY, INFINITY, INFINITY, -INFINITY);
+  o[2 * e + 1] = make_float4(-INFINITY, -INFINITY, 0.0f, 0.0f);
+}
+// Entries e, e + 1: the children of internal node y in y's order of
+// consideration (a leaf child first, A before B).
+__device__ inline void put_pair(float4* o, int e, const Node& y, int32_t split) {
+  if (y.childA >= 0 && y.childB < 0) {
+    put_entry(o, e, y.b_min, y.b_max, y.childB, split);
+    put_entry(o, e + 1, y.a_min, y.a_max, y.childA, 0);
+  } else {
+    put_entry(o, e, y.a_min, y.a_max, y.childA, split);
+    put_entry(o, e + 1, y.b_min, y.b_max, y.childB, 0);
+  }
+}
+
+// The wide form of a rebuilt body, record x from internal node x: X's leaf
+// children (A before B), then the children of its internal children (A's,
+// then B's).  With both children internal, entry 0's split word is B's
+// index.  Reads nodes x, A and B (64 bytes each), writes 128 bytes.
+__global__ __launch_bounds__(256) void k_fold_grouped(PF<const Node> nodes_f, uint32_t mi, PF<float4> wide_f) {
+  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= mi) return;
+  const Node* __restrict__ N = pick(nodes_f);
+  float4* o = pick(wide_f) + 8 * (size_t)x;
+  const Node X = N[x];
+  const bool la = X.childA < 0, lb = X.childB < 0;
+  if (la && lb) {
+    put_entry(o, 0, X.a_min, X.a_max, X.childA, 0);
+    put_entry(o, 1, X.b_min, X.b_max, X.childB, 0);
+    put_empty(o, 2);
+    put_empty(o, 3);
+  } else if (la || lb) {
+    if (la) {
+      put_entry(o, 0, X.a_min, X.a_max, X.childA, 0);
+      put_pair(o, 1, N[X.childB], 0);
+    } else {
+      put_entry(o, 0, X.b_min, X.b_max, X.childB, 0);
+      put_pair(o, 1, N[X.childA], 0);
+    }
+    put_empty(o, 3);
+  } else {
+    put_pair(o, 0, N[X.childA], X.childB);
+    put_pair(o, 2, N[X.childB], 0);
+  }
+}
+
+void Body::fold_frames(int F, const int* ks, hipStream_t s) {
+  if (!grouped || !wide_enabled) return;
+  PF<const Node> nodes{};
+  PF<float4> wide{};
+  for (int f = 0; f < F; ++f) {
+    nodes.p[f] = slot[ks[f]].nodes.p;
+    wide.p[f] = slot[ks[f]].wide.p;
+  }
+  k_fold_grouped<<<dim3(grid_for(m - 1, 256), (unsigned)F), 256, 0, s>>>(nodes, m - 1, wide);
+  CC_LAUNCHED();
 }
 
 void Body::wide_setup(int k, hipStream_t s) {

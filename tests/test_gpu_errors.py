@@ -56,12 +56,13 @@ def test_nonfinite_state_is_reported(gpu, bad):
     sim.close()
 
 
-@pytest.mark.parametrize("width", [4, 2])
-def test_traversal_overflow_is_recorded_and_reported(gpu, width):
+@pytest.mark.parametrize("width,mode", [(4, L.CC_BODY_STATIC), (2, L.CC_BODY_STATIC), (4, L.CC_BODY_REBUILD)])
+def test_traversal_overflow_is_recorded_and_reported(gpu, width, mode):
     # a 48x48 cloth resting on the 50,688-triangle capsule body after 150
     # steps: a CPU replay of the binary walk over this state finds ~200
-    # queries that defer two or more nodes at once
-    body = S.capsules(64, 14, 6, L.CC_BODY_STATIC)
+    # queries that defer two or more nodes at once (a rebuilt body: the
+    # wide records folded on the device every frame)
+    body = S.capsules(64, 14, 6, mode)
     sc = S.grid_scene("stack", 48, 1e-3, 0, body, box_depth=0.002, y0=0.93)
     sim = Simulator.from_scene(sc)
     ref = Simulator.from_scene(sc)
@@ -72,7 +73,7 @@ def test_traversal_overflow_is_recorded_and_reported(gpu, width):
     sim.set_stack_limit(1)
     sim.simulate(5)
     msg = expect_error(sim.check_finite, L.CC_ERR_TRAVERSAL)
-    say(f"[errors] width {width}, stack limit 1: {msg}")
+    say(f"[errors] width {width}, body mode {mode}, stack limit 1: {msg}")
     sim.check_finite()                    # reported once, then cleared
     sim.set_stack_limit(64)
     ref.simulate(5)

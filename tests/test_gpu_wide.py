@@ -1,9 +1,10 @@
 """The four-wide traversal form (wide.hip) against the binary walk of
 D_BVH::intersect (cudaBRT.cu:239-285): two simulators on the same scene, one
 walking each form, stay bit-identical step after step (positions, previous
-positions, contact normals), for a static body and for a body refitted every
-frame; the per-point query gives the same leaf, inside flag and projection.
-The binary walk is itself pinned to the oracle by the other GPU tests."""
+positions, contact normals), for a static body, a body refitted every frame
+and a body rebuilt every frame (its wide records folded on the device); the
+per-point query gives the same leaf, inside flag and projection.  The binary
+walk is itself pinned to the oracle by the other GPU tests."""
 import numpy as np
 import pytest
 
@@ -53,6 +54,16 @@ def test_refitted_body_wide_equals_binary(gpu):
     assert_same_run(sc, blocks=5, per_block=40)
 
 
+def test_rebuilt_body_wide_equals_binary(gpu):
+    sc = S.grid_scene("rebuild128", 128, 5e-4, 200, S.capsules(24, 12, 6, L.CC_BODY_REBUILD), box_depth=0.002,
+                      y0=0.975)
+    m = sc.body.tris.shape[0]
+    a = Simulator.from_scene(sc)
+    assert a.bvh_width() == (4, m - 1, 64), a.bvh_width()    # one record per internal node
+    a.close()
+    assert_same_run(sc, blocks=5, per_block=40)
+
+
 def test_queries_wide_equals_binary(gpu):
     sc = S.small(side=16)
     a, b = pair(sc)
@@ -81,11 +92,14 @@ def test_width_switch_and_rebuild(gpu):
         a.set_bvh_width(3)
     a.close()
     g = Simulator.from_scene(S.garment(side=16))
-    assert g.bvh_width()[0] == 2    # rebuilt every frame: always the binary walk
+    w, nw, bound = g.bvh_width()    # rebuilt every frame: records folded on the device
+    assert w == 4 and nw == g.bvh_size - 1 and bound == 64, (w, nw, bound)
+    g.set_bvh_width(2)
+    assert g.bvh_width()[0] == 2
     g.close()
 
 
-def crossed_body(pairs):
+def crossed_body(pairs, mode=L.CC_BODY_STATIC):
     """2 * pairs + 1 triangles whose boxes share the centre (0, 0, 0), so one
     Morton code and a tree interleaved by index: per pair one triangle flat in
     y and one flat in x, then one whose box is the cube [-1, 1]^3.  A point
@@ -95,7 +109,8 @@ def crossed_body(pairs):
     the last leaf lies at the bottom of the rightmost path; an emulation of
     the four-wide walk over the oracle's tree visits 3,393 records for such a
     point, with up to 18 entries deferred at once.  (With 2^k + 1 leaves the
-    last one would be the root's own leaf child, found at once.)"""
+    last one would be the root's own leaf child, found at once.)  A rebuilt
+    body gets one capsule over all its vertices and no motion."""
     v = []
     for _ in range(pairs):
         v += [(-1.0, -0.1, -1.0), (1.0, 0.1, -1.0), (0.0, 0.0, 1.0)]
@@ -103,15 +118,22 @@ def crossed_body(pairs):
     v += [(-1.0, -1.0, -1.0), (1.0, 1.0, -1.0), (0.0, 0.0, 1.0)]
     verts = np.array(v, np.float32)
     tris = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
-    return S.Body(verts, tris, np.zeros_like(verts), L.CC_BODY_STATIC, False)
+    caps = None
+    if mode == L.CC_BODY_REBUILD:
+        caps = {"vstart": np.array([0, len(v)], np.uint32), "p0": np.zeros((1, 3), np.float32),
+                "p1": np.array([[0.0, 0.0, 1.0]], np.float32), "radius": np.ones(1, np.float32),
+                "phase": np.zeros(1, np.float32)}
+    return S.Body(verts, tris, np.zeros_like(verts), mode, False, caps)
 
 
-def test_queries_deep_stack_wide_equals_binary(gpu):
+@pytest.mark.parametrize("mode", [L.CC_BODY_STATIC, L.CC_BODY_REBUILD])
+def test_queries_deep_stack_wide_equals_binary(gpu, mode):
     # the box depth is 0 so that a box's centre is its triangle's centroid
-    sc = S.grid_scene("crossed", 8, 1e-3, 10, crossed_body(3000), box_depth=0.0)
+    sc = S.grid_scene("crossed", 8, 1e-3, 10, crossed_body(3000, mode), box_depth=0.0)
     a, b = pair(sc)
     w, nw, bound = a.bvh_width()
-    # more deferrals than the LDS part of the stack holds (16 entries)
+    # more deferrals than the LDS part of the stack holds (16 entries); a
+    # rebuilt body reports the stack limit
     assert w == 4 and 16 < bound <= 64, (w, nw, bound)
     rng = np.random.default_rng(11)
     z = rng.uniform(-0.9, 0.9, 96)
