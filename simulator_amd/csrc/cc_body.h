@@ -232,10 +232,12 @@ struct Body {
 
   // set-up: host arrays, vertices stride 3 (already extended); a moving
   // body gets `slots` frame slots and `lanes` update lanes, a rebuilt one
-  // built bottom-up `chain` frames per chain of launches
+  // built bottom-up `chain` frames per chain of launches (allow_bottom_up
+  // false: a rebuilt body of any size takes the Karras searches and the
+  // refit, the path of bodies beyond the bottom-up build's leaf limit)
   void init(const float* verts3, uint32_t nv, const uint32_t* tris, uint32_t nf, uint32_t mode, float depth,
             const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s,
-            int chain = 1);
+            int chain = 1, bool allow_bottom_up = true);
 
   // launches, asynchronous on s
   void deform(int k, uint64_t frame, hipStream_t s);   // rest -> slot k pose of `frame`

@@ -368,7 +368,8 @@ void Lane::alloc(uint32_t m, bool full_lane, bool bottom_up, hipStream_t s) {
 }
 
 void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, uint32_t mode_, float depth_,
-                const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s, int chain_) {
+                const cc_capsules* caps, const MotionParams& mp, int slots, int lanes, hipStream_t s, int chain_,
+                bool allow_bottom_up) {
   CC_REQUIRE(mode_ <= CC_BODY_REBUILD, "body: unknown mode");
   mode = mode_;
   nv = nv_;
@@ -413,7 +414,7 @@ void Body::init(const float* v3, uint32_t nv_, const uint32_t* t3, uint32_t nf, 
     S.motion.alloc(12 + (size_t)ncaps);
     S.motion.zero(s);
   }
-  bottom_up = mode == CC_BODY_REBUILD && m >= 2 && m <= LV_MAX_LEAVES;
+  bottom_up = mode == CC_BODY_REBUILD && m >= 2 && m <= LV_MAX_LEAVES && allow_bottom_up;
   if (bottom_up) {
     BodyDesc d{};
     d.nv = nv;

@@ -422,8 +422,12 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       int chain = 2;
       if (const char* e = std::getenv("CC_BUILD_FRAMES")) chain = std::atoi(e);
       CC_REQUIRE(chain >= 1 && chain <= MAX_FRAMES, "cc_sim_create: CC_BUILD_FRAMES must be 1..4");
+      // 0: rebuilt bodies of any size take the Karras searches and the
+      // refit (the path of bodies beyond 2^21 triangles)
+      bool bottom_up = true;
+      if (const char* e = std::getenv("CC_REBUILD_BOTTOM_UP")) bottom_up = std::atoi(e) != 0;
       S.body.init(bv.data(), bd->nv, bd->tris, bd->nf, bd->mode, params->box_depth, bd->caps, mp,
-                  2 * S.graph_steps, lanes, s, chain);
+                  2 * S.graph_steps, lanes, s, chain, bottom_up);
       // frame 0 into slot 0; a refitted body keeps the topology built from
       // the frame-0 pose in every slot
       S.body.deform(0, 0, s);

@@ -64,6 +64,34 @@ def test_rebuilt_body_wide_equals_binary(gpu):
     assert_same_run(sc, blocks=5, per_block=40)
 
 
+def test_karras_rebuild_path_equals_bottom_up(gpu, monkeypatch):
+    # A rebuilt body beyond 2^21 triangles takes the Karras searches, the
+    # refit and then the fold; CC_REBUILD_BOTTOM_UP=0 forces that path on a
+    # small body: the same trees and the same run as the bottom-up build,
+    # bit for bit, both walked through their folded records
+    sc = S.grid_scene("rebuild96", 96, 5e-4, 200, S.capsules(24, 12, 6, L.CC_BODY_REBUILD), box_depth=0.002,
+                      y0=0.975)
+    monkeypatch.setenv("CC_REBUILD_BOTTOM_UP", "0")
+    k = Simulator.from_scene(sc)
+    monkeypatch.delenv("CC_REBUILD_BOTTOM_UP")
+    b = Simulator.from_scene(sc)
+    assert k.bvh_width() == b.bvh_width() == (4, sc.body.tris.shape[0] - 1, 64)
+    contacts = 0
+    for blk in range(4):
+        k.simulate(40)
+        b.simulate(40)
+        for x, y in zip(k.get_state(), b.get_state()):
+            np.testing.assert_array_equal(x, y, err_msg=f"block {blk}")
+        tk, tb = k.bvh_export(), b.bvh_export()
+        for key in tk:
+            np.testing.assert_array_equal(tk[key], tb[key], err_msg=f"block {blk}: {key}")
+        contacts += int(np.any(k.get_state()[2] != 0, axis=1).sum())
+    say(f"[wide] Karras rebuild path: {contacts} contacts over 4 blocks, equal to the bottom-up build")
+    assert contacts > 0
+    k.close()
+    b.close()
+
+
 def test_queries_wide_equals_binary(gpu):
     sc = S.small(side=16)
     a, b = pair(sc)
