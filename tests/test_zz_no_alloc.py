@@ -1,8 +1,8 @@
 """cc_collide_points_noalloc (no device allocation; its traversal spill area
 is a fixed device array per GPU) against cc_collide_points, bit for bit:
 two calls in a row on device buffers the test owns, the second with fewer
-points, over a static body and over a refitted body between steps; sizes
-beyond the maximum are rejected."""
+points, over a static body and over a refitted or rebuilt body between
+steps; sizes beyond the maximum are rejected."""
 import ctypes as C
 
 import numpy as np
@@ -64,15 +64,17 @@ def test_noalloc_query_equals_the_allocating_one_static_body(gpu):
     sim.close()
 
 
-def test_noalloc_query_equals_the_allocating_one_refitted_body(gpu):
-    body = S.capsules(64, 14, 6, L.CC_BODY_REFIT)
+@pytest.mark.parametrize("mode", [L.CC_BODY_REFIT, L.CC_BODY_REBUILD])
+def test_noalloc_query_equals_the_allocating_one_moving_body(gpu, mode):
+    body = S.capsules(64, 14, 6, mode)
     sc = S.grid_scene("noalloc", 48, 1e-3, 0, body, box_depth=0.002, y0=0.93)
     sim = Simulator.from_scene(sc)
     sim.simulate(37)                       # single steps: poses issued ahead on the update lanes
     v = sim.body_vertices()
     rng = np.random.default_rng(5)
     pts = (v[rng.integers(0, v.shape[0], 20000)] + rng.normal(scale=3e-3, size=(20000, 3))).astype(np.float32)
-    assert same_as_existing(sim, pts, "refitted body, frame 37") > 1000
+    kind = "refitted" if mode == L.CC_BODY_REFIT else "rebuilt"
+    assert same_as_existing(sim, pts, f"{kind} body, frame 37") > 1000
     sim.simulate(1)
-    assert same_as_existing(sim, pts[:5000], "refitted body, frame 38, fewer points") > 200
+    assert same_as_existing(sim, pts[:5000], f"{kind} body, frame 38, fewer points") > 200
     sim.close()
