@@ -179,9 +179,10 @@ __device__ inline WideRoot load_wide_root(const BvhView& b) {
 //    children of its internal children.  When both children A and B are
 //    internal, entries 0-1 are A's children and 2-3 B's, and entry 0's
 //    split word is B's index: while the first candidate lies in A's pair,
-//    the walk defers B's record instead of B's entries.  Then every visit
-//    defers at most one node per binary level it spans, as the binary walk
-//    does, so the stack stays within the tree depth (below 64).
+//    the walk defers B's record instead of B's two entries (or the one of
+//    them that holds p).  Then every visit defers at most one node per
+//    binary level it spans, as the binary walk does, so the stack stays
+//    within the tree depth (below 64).
 __device__ inline void wstk_put(const Spill& sp, int k, int32_t v) {
   if (k < WSTK_LDS)
     sp.lds[k * WSTK_LANES] = v;
@@ -222,10 +223,13 @@ __device__ inline int32_t wide_visit(f3 p, float4 q0, float4 q1, float4 q2, floa
   if constexpr (GROUPED) {
     // A split record whose first candidate is in entries 0-1: entries 2-3
     // become one candidate, the record of their parent (the split word;
-    // the root is no node's child, so a split is never 0).
+    // the root is no node's child, so a split is never 0) when both hold
+    // p, else the one that does (the same next step as that record's
+    // visit, one visit fewer).
     const int32_t split = __float_as_int(q1.w);
     const bool in_a = (split != 0) & (i0 | i1);
-    c2 = in_a ? split : c2;
+    const int32_t b_next = (i2 & i3) ? split : (i2 ? c2 : c3);
+    c2 = in_a ? b_next : c2;
     i2 = in_a ? (i2 | i3) : i2;
     i3 = i3 & !in_a;
   }

@@ -66,7 +66,9 @@ def walk(recs, p):
         ins = [in_box(b, p) for b, _ in ents]
         links = [l for _, l in ents]
         if split != 0 and (ins[0] or ins[1]):
-            ins[2], links[2], ins[3] = ins[2] or ins[3], split, False
+            # B's record when both of B's children hold p, else the one that does
+            b_next = split if (ins[2] and ins[3]) else (links[2] if ins[2] else links[3])
+            ins[2], links[2], ins[3] = ins[2] or ins[3], b_next, False
         cand = [e for e in range(4) if ins[e]]
         if not cand:
             if not stack:
