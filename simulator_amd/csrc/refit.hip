@@ -237,10 +237,10 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
   const uint32_t tid = threadIdx.x;
   const uint32_t H = m > 1 ? chunk_h[blockIdx.x] : 0;
 
-  // Loads in two rounds, each issued in full before its first use (indices
+  // Loads in two batches, each issued in full before its first use (indices
   // clamped instead of guarded: a guard per entry lets the compiler sink
   // each load into its branch, one memory round trip per entry).
-  // Round 1: the thread's leaves (triangle, flags, parent) and internal
+  // First batch: the thread's leaves (triangle, flags, parent) and internal
   // nodes (links, flags).
   const int32_t lmax = c1 - 1, xmax = max(i1 - 1, 0);
   const int32_t tmax = max((int32_t)m - 2, 0);                  // last entry of top_pos
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refit_chunks(
     }
     if (x >= c1) lf[r] = 0;
   }
-  // Round 2: the leaves' vertices, and for every box that goes directly to
+  // Second batch: the leaves' vertices, and for every box that goes directly to
   // a top node, that node's top-pass record and (four-wide form) the wide
   // entry of its slot.
   float vtx[RC_PER_THREAD][9];
