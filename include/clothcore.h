@@ -402,7 +402,8 @@ typedef struct cc_timing {
   double verlet_us;    /* fused Verlet + spring + collision kernel      */
   double body_us;      /* deformation kernel                            */
   double refit_us;     /* refit kernels                                 */
-  double build_us;     /* rebuild kernels (boxes, sort, Karras) summed  */
+  double build_us;     /* rebuild kernels (boxes, sort, tree, four-wide
+                          fold) summed                                   */
   uint32_t steps;
 } cc_timing;
 int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out);

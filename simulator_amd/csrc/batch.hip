@@ -129,9 +129,8 @@ struct Batch {
     c0.alloc(n3);
     CC_HIP(hipMemcpyAsync(s0.p, state[0].p, n6 * sizeof(float), hipMemcpyDeviceToDevice, stream));
     CC_HIP(hipMemcpyAsync(c0.p, contact.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    hipEvent_t e0, e1;
-    CC_HIP(hipEventCreate(&e0));
-    CC_HIP(hipEventCreate(&e1));
+    const Events<2> ev;
+    const hipEvent_t e0 = ev[0], e1 = ev[1];
     const bool fused = bodies.moving && bodies.step.n_sort;
     float best = 1e30f;
     for (int T : TILES)
@@ -160,8 +159,6 @@ struct Batch {
           order = ord;
         }
       }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     cur = 0;
     CC_HIP(hipMemcpyAsync(state[0].p, s0.p, n6 * sizeof(float), hipMemcpyDeviceToDevice, stream));
     CC_HIP(hipMemcpyAsync(contact.p, c0.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, stream));

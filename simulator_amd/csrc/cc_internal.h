@@ -128,6 +128,27 @@ inline void fill_on(void* dst, int value, size_t bytes, hipStream_t s) {
   CC_HIP(hipStreamSynchronize(s));
 }
 
+// N events of a measurement, destroyed on every way out of it (the HIP calls
+// between their creation and the end of the measurement throw on error; if
+// a creation throws, the events made before it are destroyed too).
+struct Ev {
+  hipEvent_t h = nullptr;
+  Ev() = default;
+  Ev(const Ev&) = delete;
+  Ev& operator=(const Ev&) = delete;
+  ~Ev() {
+    if (h) (void)hipEventDestroy(h);
+  }
+};
+template <int N>
+struct Events {
+  Ev e[N];
+  Events() {
+    for (Ev& x : e) CC_HIP(hipEventCreate(&x.h));
+  }
+  hipEvent_t operator[](int i) const { return e[i].h; }
+};
+
 // ------------------------------------------------------------ physics pod
 struct Params {
   float dt, mass, gx, gy, gz, damp, kd, k0, k1, k2, rest_scale, eps, box_depth;

@@ -988,8 +988,7 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     hipStream_t s = S.stream;
     CC_HIP(hipDeviceSynchronize());
     std::memset(out, 0, sizeof(*out));
-    std::vector<hipEvent_t> ev(5);
-    for (auto& e : ev) CC_HIP(hipEventCreate(&e));
+    const Events<5> ev;
     double t_body = 0, t_upd = 0, t_ver = 0;
     // the work of a step one kernel group at a time: the body update of the
     // next frame, then the step of this frame
@@ -1024,7 +1023,6 @@ int cc_sim_time_kernels(cc_sim* sim, int nsteps, cc_timing* out) {
     CC_HIP(hipEventSynchronize(ev[4]));
     float w;
     CC_HIP(hipEventElapsedTime(&w, ev[0], ev[4]));
-    for (auto& e : ev) (void)hipEventDestroy(e);
     out->steps = (uint32_t)nsteps;
     out->body_us = 1e3 * t_body / nsteps;
     if (S.body.mode == CC_BODY_REFIT) out->refit_us = 1e3 * t_upd / nsteps;
