@@ -179,3 +179,43 @@ def test_queries_deep_stack_wide_equals_binary(gpu, mode):
     assert np.all(ra[0][-3:] == -1)
     a.close()
     b.close()
+
+
+def one_capsule(nv):
+    """One capsule over all nv vertices and no motion: a rebuilt body that
+    keeps its rest pose."""
+    return {"vstart": np.array([0, nv], np.uint32), "p0": np.zeros((1, 3), np.float32),
+            "p1": np.array([[0.0, 0.0, 1.0]], np.float32), "radius": np.ones(1, np.float32),
+            "phase": np.zeros(1, np.float32)}
+
+
+@pytest.mark.parametrize("case", ["one", "two", "three", "same300"])
+def test_small_and_repeated_rebuilt_bodies(gpu, case):
+    # The device fold at its edges: one triangle (no record), two and three
+    # (records of leaves only, one split-free record with a pair), and 300
+    # copies of one triangle (all Morton codes equal, the tree ordered by
+    # index): the tree against the oracle's build, and the point queries
+    # against the oracle and the binary walk, bit for bit
+    from helpers import assert_bvh_equal, collider_sim, oracle_bvh
+    import oracle as O
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.5], [0.5, 1.5, 0.2]], np.float32)
+    tris = np.array({"one": [[0, 1, 2]], "two": [[0, 1, 2], [1, 3, 2]],
+                     "three": [[0, 1, 2], [1, 3, 2], [2, 3, 4]], "same300": [[0, 1, 2]] * 300}[case], np.uint32)
+    a = collider_sim(v, tris, mode=L.CC_BODY_REBUILD, caps=one_capsule(len(v)))
+    b = collider_sim(v, tris, mode=L.CC_BODY_REBUILD, caps=one_capsule(len(v)))
+    b.set_bvh_width(2)
+    m = tris.shape[0]
+    assert a.bvh_width()[:2] == ((4, m - 1) if m >= 2 else (2, 0)), a.bvh_width()
+    d = oracle_bvh(a.body_vertices(), tris, 0.01)
+    assert_bvh_equal(a.bvh_export(), d)
+    rng = np.random.default_rng(17)
+    pts = np.concatenate([rng.uniform([-0.2, -0.2, -0.1], [1.2, 1.6, 0.6], size=(3000, 3)),
+                          np.array([[0.2, 0.2, -0.005], [0.9, 0.9, 0.45], [0.3, 0.3, 0.0]])]).astype(np.float32)
+    got, alt = a.collide_points(pts), b.collide_points(pts)
+    want = O.collide_points(O.BvhHandle(d), pts)
+    for g, w, x in zip(got, want, alt):
+        np.testing.assert_array_equal(g, w)
+        np.testing.assert_array_equal(g, x)
+    say(f"[wide] rebuilt body '{case}': {m} triangles, {int((got[0] >= 0).sum())} of {pts.shape[0]} queries hit")
+    a.close()
+    b.close()
