@@ -408,15 +408,17 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       mp.pz = bd->pivot[2];
       mp.dt = (double)params->dt;
       // Update lanes and frames per chain of launches.  A refitted body:
-      // three lanes of two frames (the 501k-triangle body, A/B in one
-      // process per setting on the same GPU: 44.6-44.8 us per step in graph
-      // blocks, frames 50-530, against 49.3-49.5 with two lanes and
-      // 52.6-53.5 with four; one call per frame 52.4-52.7 against 52.2-52.4
-      // with two).  A rebuilt body, whose update is a chain of 17 small
-      // launches: two lanes of three frames (111 x 111 over the
-      // 50,688-triangle body: 35.4-35.6 us per step over 500 steps through
-      // the graphs against 37.0-37.5 with three lanes of two, 40.0-40.1
-      // against 41.0-42.0 in contact; one call per frame the same).
+      // three lanes (the 501k-triangle body, A/B in one process per setting
+      // on the same GPU: 44.6-44.8 us per step in graph blocks, frames
+      // 50-530, against 49.3-49.5 with two lanes and 52.6-53.5 with four;
+      // one call per frame 52.4-52.7 against 52.2-52.4 with two) of three
+      // frames (20 single steps 33.8-34.5 us against 34.5-35.4 with two,
+      // 500 steps 41.7-42.0 against 42.2-42.6, one call per frame the same).
+      // A rebuilt body, whose update is a chain of 17 small launches: two
+      // lanes of three frames (111 x 111 over the 50,688-triangle body:
+      // 35.4-35.6 us per step over 500 steps through the graphs against
+      // 37.0-37.5 with three lanes of two, 40.0-40.1 against 41.0-42.0 in
+      // contact; one call per frame the same).
       const bool rebuilt = bd->mode == CC_BODY_REBUILD;
       int lanes = rebuilt ? 2 : 3;
       if (const char* e = std::getenv("CC_BODY_LANES")) lanes = std::atoi(e);
@@ -424,7 +426,7 @@ int cc_sim_create(const cc_cloth_desc* cloth, const cc_body_desc* bd, const cc_p
       if (const char* e = std::getenv("CC_GRAPH_STEPS")) S.graph_steps = std::atoi(e);
       CC_REQUIRE(S.graph_steps >= 2 && S.graph_steps % 2 == 0 && 2 * S.graph_steps <= MAX_SLOTS,
                  "cc_sim_create: CC_GRAPH_STEPS must be even, 2..MAX_SLOTS/2");
-      int chain = rebuilt ? 3 : 2;
+      int chain = 3;
       if (const char* e = std::getenv("CC_BUILD_FRAMES")) chain = std::atoi(e);
       CC_REQUIRE(chain >= 1 && chain <= MAX_FRAMES, "cc_sim_create: CC_BUILD_FRAMES must be 1..4");
       // 0: rebuilt bodies of any size take the Karras searches and the
