@@ -74,3 +74,25 @@ def test_any_call_pattern_equals_one_call_per_frame(gpu, mode):
     assert in_contact > 0
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("mode", [L.CC_BODY_REFIT, L.CC_BODY_REBUILD])
+@pytest.mark.parametrize("lanes,frames", [(1, 1), (2, 2), (3, 1), (4, 4)])
+def test_lane_and_chain_settings_change_no_result(gpu, mode, lanes, frames, monkeypatch):
+    # CC_BODY_LANES / CC_BUILD_FRAMES reshape the update pipeline (how many
+    # streams, how many frames per chain of launches); graph blocks, runs
+    # that end inside a block and single steps give the default's results
+    sc = _scene(mode)
+    ref = Simulator.from_scene(sc)
+    monkeypatch.setenv("CC_BODY_LANES", str(lanes))
+    monkeypatch.setenv("CC_BUILD_FRAMES", str(frames))
+    sim = Simulator.from_scene(sc)
+    monkeypatch.delenv("CC_BODY_LANES")
+    monkeypatch.delenv("CC_BUILD_FRAMES")
+    for n in (48, 7, 1, 1, 30, 24, 5):
+        ref.simulate(n)
+        sim.simulate(n)
+        _same_state(sim, ref, f"lanes {lanes}, frames {frames}, mode {mode}, frame {sim.frame}")
+    say(f"[patterns] {lanes} lanes of {frames} frames, mode {mode}: the default's {sim.frame} frames")
+    ref.close()
+    sim.close()
